@@ -1,0 +1,413 @@
+// kernels_cost.hip -- per-pixel descriptors, matching-cost volumes and pyrDown.
+//
+// Descriptors (kPrep): one uint2 per pixel per view.  Census: x = 5x5 census
+// word (sgm_kernels.cu:251-298).  BT: the reference's per-channel BT interval
+// (u, min, max of the two half-pixel neighbours, sgm_kernels.cu:1002-1028) is
+// a property of one pixel, so it is computed once per pixel and packed as
+// x = gradient channel, y = intensity channel, each (v | v0<<8 | v1<<16).
+// A BT cost is then two interval distances (sgm_kernels.cu:1025-1027, :1042).
+//
+// Volumes are int16 [y][x][Dp], labels innermost; every thread produces 8
+// consecutive labels of one pixel and writes them with one 16-byte store.
+#include "device_common.h"
+#include "internal.h"
+
+namespace pyr {
+namespace {
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return imin(imax(v, lo), hi); }
+
+// ---- BT helpers ----------------------------------------------------------
+__device__ __forceinline__ uint32_t btInterval(int v, int l, int r)
+{
+    const int hl = (v + l) / 2, hr = (v + r) / 2;
+    const int v0 = imin(imin(hl, hr), v), v1 = imax(imax(hl, hr), v);
+    return (uint32_t)v | ((uint32_t)v0 << 8) | ((uint32_t)v1 << 16);
+}
+
+__device__ __forceinline__ int btChannel(uint32_t a, uint32_t b)
+{
+    const int u = a & 0xFF, u0 = (a >> 8) & 0xFF, u1 = (a >> 16) & 0xFF;
+    const int v = b & 0xFF, v0 = (b >> 8) & 0xFF, v1 = (b >> 16) & 0xFF;
+    const int c0 = imax(imax(0, u - v1), v0 - u);
+    const int c1 = imax(imax(0, v - u1), u0 - v);
+    return imin(c0, c1);
+}
+
+__device__ __forceinline__ int btCost(uint2 l, uint2 r)
+{
+    return imin(255, btChannel(l.x, r.x) + (btChannel(l.y, r.y) >> 2));
+}
+
+// sgm_kernels.cu:94-118
+__device__ __forceinline__ int btGrad(const uint8_t* img, int step, int W, int H, int x, int y,
+                                      int ftzero)
+{
+    const int xp = imax(x - 1, 0), xn = imin(x + 1, W - 1);
+    const int yp = imax(y - 1, 0), yn = imin(y + 1, H - 1);
+    const uint8_t* r = img + (size_t)y * step;
+    const uint8_t* u = img + (size_t)yp * step;
+    const uint8_t* d = img + (size_t)yn * step;
+    int g = ((int)r[xn] - (int)r[xp]) * 2 + ((int)u[xn] - (int)u[xp]) + ((int)d[xn] - (int)d[xp]);
+    g = clampi(g, -ftzero, ftzero);
+    return g + ftzero;
+}
+
+// ---- descriptors + projection-key reset ----------------------------------
+__global__ __launch_bounds__(256) void kPrep(const uint8_t* left, int lstep, const uint8_t* right,
+                                             int rstep, uint2* desc, uint32_t* keys, int W, int H,
+                                             int census, int cap)
+{
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int z = blockIdx.z;
+    if (x >= W || y >= H) return;
+    const uint8_t* img = z ? right : left;
+    const int step = z ? rstep : lstep;
+    const size_t p = (size_t)y * W + x;
+    uint2 out;
+    if (census) {
+        const int c = img[(size_t)y * step + x];
+        uint32_t cv = 0;
+#pragma unroll
+        for (int dy = -2; dy <= 2; ++dy) {
+            const uint8_t* row = img + (size_t)clampi(y + dy, 0, H - 1) * step;
+#pragma unroll
+            for (int dx = -2; dx <= 2; ++dx)
+                cv = (cv << 1) | (uint32_t)(row[clampi(x + dx, 0, W - 1)] < c);
+        }
+        out = make_uint2(cv, 0u);
+    } else {
+        const int ftzero = imax(cap, 15) | 1;
+        const int xm = imax(x - 1, 0), xq = imin(x + 1, W - 1);
+        const int gm = btGrad(img, step, W, H, xm, y, ftzero);
+        const int g0 = btGrad(img, step, W, H, x, y, ftzero);
+        const int gq = btGrad(img, step, W, H, xq, y, ftzero);
+        const uint8_t* row = img + (size_t)y * step;
+        out = make_uint2(btInterval(g0, gm, gq), btInterval(row[x], row[xm], row[xq]));
+    }
+    desc[(size_t)z * W * H + p] = out;
+    if (z == 0) keys[p] = 0xFFFFFFFFu;
+}
+
+__device__ __forceinline__ void store8(int16_t* dst, const int (&v)[8])
+{
+    uint4 o;
+    o.x = (uint32_t)(uint16_t)v[0] | ((uint32_t)v[1] << 16);
+    o.y = (uint32_t)(uint16_t)v[2] | ((uint32_t)v[3] << 16);
+    o.z = (uint32_t)(uint16_t)v[4] | ((uint32_t)v[5] << 16);
+    o.w = (uint32_t)(uint16_t)v[6] | ((uint32_t)v[7] << 16);
+    *reinterpret_cast<uint4*>(dst) = o;
+}
+
+// ---- Census cost over the full label range (sgm_kernels.cu:303-327) --------
+// Block: 256 threads = NPX pixels x (Dp/8) label octets of one row segment; the
+// right-view census words the segment can reach are staged in LDS.
+__global__ __launch_bounds__(256) void kCensusCost(const uint2* desc, int16_t* C, int W, int H,
+                                                   int D, int Dp, int minD)
+{
+    extern __shared__ uint32_t s_r[];
+    const int tpp = Dp >> 3;
+    const int npx = 256 / tpp;
+    const int y = blockIdx.y;
+    const int x0 = blockIdx.x * npx;
+    const int base2 = x0 - minD - (Dp - 1);
+    const int nR = npx + Dp - 1;
+    const uint2* L = desc + (size_t)y * W;
+    const uint2* R = desc + (size_t)W * H + (size_t)y * W;
+    for (int i = threadIdx.x; i < nR; i += 256) {
+        const int xx = base2 + i;
+        s_r[i] = (xx >= 0 && xx < W) ? R[xx].x : 0u;
+    }
+    __syncthreads();
+    const int px = threadIdx.x / tpp;
+    const int d0 = (threadIdx.x - px * tpp) * 8;
+    const int x = x0 + px;
+    if (x >= W) return;
+    const uint32_t cl = L[x].x;
+    int v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int d = d0 + i;
+        const int x2 = x - d - minD;
+        v[i] = d >= D ? kMax
+                      : ((x2 >= 0 && x2 < W) ? __popc(cl ^ s_r[x2 - base2]) : 24);
+    }
+    store8(C + ((size_t)y * W + x) * Dp + d0, v);
+}
+
+// ---- refinement labels (sgm_kernels.cu:955-1000) ----------------------------
+struct RefineIn {
+    const int16_t* pdisp;
+    const float* pconf;
+    int pdpitch, pcpitch, PW, PH;
+    float minConf;
+};
+
+__device__ __forceinline__ void refineLabel(const RefineIn& r, int x, int y, int D, int minD,
+                                            int maxD, int& base, int& valid)
+{
+    const int px = imin(x >> 1, r.PW - 1), py = imin(y >> 1, r.PH - 1);
+    const int raw = r.pdisp[(size_t)py * r.pdpitch + px];
+    const float cf = r.pconf[(size_t)py * r.pcpitch + px];
+    const int pinv = (int)(int16_t)((minD - 1) * 16);
+    if (raw == pinv || cf < r.minConf) {
+        base = minD;
+        valid = 0;
+        return;
+    }
+    const int p16 = raw * 2;
+    const int center = (p16 + 8) >> 4;
+    const int res = p16 - center * 16;
+    int b = center + (res < 0 ? -(D / 2) : -(D / 2) + 1);
+    const int largest = imin(maxD, x) - D + 1;
+    b = largest >= minD ? imin(imax(b, minD), largest) : minD;
+    base = b;
+    valid = 1;
+}
+
+// ---- Census cost at refinement levels (sgm_kernels.cu:1112-1133) ------------
+__global__ __launch_bounds__(256) void kCensusLocal(const uint2* desc, RefineIn r, int32_t* labels,
+                                                    int16_t* C, int W, int H, int D, int Dp,
+                                                    int minD, int maxD)
+{
+    const int tpp = Dp >> 3;
+    const int npx = 256 / tpp;
+    const int y = blockIdx.y;
+    const int px = threadIdx.x / tpp;
+    const int d0 = (threadIdx.x - px * tpp) * 8;
+    const int x = blockIdx.x * npx + px;
+    if (x >= W) return;
+    int base, valid;
+    refineLabel(r, x, y, D, minD, maxD, base, valid);
+    const size_t p = (size_t)y * W + x;
+    if (d0 == 0) labels[p] = (base & 0xFFFF) | (valid << 16);
+    const uint32_t cl = desc[p].x;
+    const uint2* R = desc + (size_t)W * H + (size_t)y * W;
+    const int lo = imax(minD, x - W + 1), hi = imin(maxD, x);
+    int v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int d = d0 + i;
+        const int real = base + d;
+        const bool ok = valid && d < D && real >= lo && real <= hi;
+        v[i] = ok ? __popc(cl ^ R[x - real].x) : kMax;
+    }
+    store8(C + p * Dp + d0, v);
+}
+
+// ---- BT cost + horizontal window sum (sgm_kernels.cu:139-239, :354-379) ------
+// Block: one 64-pixel row segment x 64 labels.  BT costs of the segment and its
+// +-half halo (clamped columns) go to LDS as bytes; each thread then sums the
+// window for 8 labels of one pixel and saturates at SHRT_MAX.
+constexpr int kSegX = 64;
+constexpr int kChunkD = 64;
+
+__global__ __launch_bounds__(256) void kBTRow(const uint2* desc, int16_t* Hs, int W, int H,
+                                              int Dp, int minD, int half)
+{
+    extern __shared__ uint8_t s_bt[];
+    const int dc = imin(Dp, kChunkD);
+    const int y = blockIdx.y;
+    const int x0 = blockIdx.x * kSegX;
+    const int dc0 = blockIdx.z * dc;
+    const int ext = kSegX + 2 * half;
+    const uint2* L = desc + (size_t)y * W;
+    const uint2* R = desc + (size_t)W * H + (size_t)y * W;
+    for (int i = threadIdx.x; i < ext * dc; i += 256) {
+        const int o = i / dc;
+        const int dl = i - o * dc;
+        const int xx = clampi(x0 - half + o, 0, W - 1);
+        const int x2 = xx - (dc0 + dl) - minD;
+        s_bt[i] = (x2 >= 0 && x2 < W) ? (uint8_t)btCost(L[xx], R[x2]) : (uint8_t)255;
+    }
+    __syncthreads();
+    const int tpp = dc >> 3;
+    for (int t = threadIdx.x; t < kSegX * tpp; t += 256) {
+        const int px = t / tpp;
+        const int dl0 = (t - px * tpp) * 8;
+        const int x = x0 + px;
+        if (x >= W) continue;
+        int v[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = 0;
+        for (int k = 0; k <= 2 * half; ++k) {
+            const uint8_t* b = s_bt + (px + k) * dc + dl0;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) v[i] += b[i];
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = imin(kMax, v[i]);
+        store8(Hs + ((size_t)y * W + x) * Dp + dc0 + dl0, v);
+    }
+}
+
+// ---- vertical window sum (sgm_kernels.cu:390-416) ----------------------------
+__global__ __launch_bounds__(256) void kBoxV(const int16_t* Hs, int16_t* C, int W, int H, int Dp,
+                                             int half)
+{
+    const size_t oct = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t nOct = (size_t)W * H * (Dp >> 3);
+    if (oct >= nOct) return;
+    const size_t p = oct / (Dp >> 3);
+    const int d0 = (int)(oct - p * (Dp >> 3)) * 8;
+    const int y = (int)(p / W);
+    const int x = (int)(p - (size_t)y * W);
+    int v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = 0;
+    for (int k = -half; k <= half; ++k) {
+        const int yy = clampi(y + k, 0, H - 1);
+        const uint4 w = *reinterpret_cast<const uint4*>(Hs + ((size_t)yy * W + x) * Dp + d0);
+        v[0] += (int16_t)(w.x & 0xFFFF); v[1] += (int)w.x >> 16;
+        v[2] += (int16_t)(w.y & 0xFFFF); v[3] += (int)w.y >> 16;
+        v[4] += (int16_t)(w.z & 0xFFFF); v[5] += (int)w.z >> 16;
+        v[6] += (int16_t)(w.w & 0xFFFF); v[7] += (int)w.w >> 16;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = imin(kMax, v[i]);
+    store8(C + p * Dp + d0, v);
+}
+
+// ---- BT window cost at refinement levels (sgm_kernels.cu:1075-1110) ----------
+__global__ __launch_bounds__(256) void kBTLocal(const uint2* desc, RefineIn r, int32_t* labels,
+                                                int16_t* C, int W, int H, int D, int Dp, int minD,
+                                                int maxD, int half)
+{
+    const int tpp = Dp >> 3;
+    const int npx = 256 / tpp;
+    const int y = blockIdx.y;
+    const int px = threadIdx.x / tpp;
+    const int d0 = (threadIdx.x - px * tpp) * 8;
+    const int x = blockIdx.x * npx + px;
+    if (x >= W) return;
+    int base, valid;
+    refineLabel(r, x, y, D, minD, maxD, base, valid);
+    const size_t p = (size_t)y * W + x;
+    if (d0 == 0) labels[p] = (base & 0xFFFF) | (valid << 16);
+    const int lo = imax(minD, x - W + 1), hi = imin(maxD, x);
+    const uint2* Ld = desc;
+    const uint2* Rd = desc + (size_t)W * H;
+    bool ok[8];
+    int sum[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int real = base + d0 + i;
+        ok[i] = valid && d0 + i < D && real >= lo && real <= hi;
+        sum[i] = 0;
+    }
+    for (int dy = -half; dy <= half; ++dy) {
+        const int yy = clampi(y + dy, 0, H - 1);
+        const uint2* Lrow = Ld + (size_t)yy * W;
+        const uint2* Rrow = Rd + (size_t)yy * W;
+        for (int dx = -half; dx <= half; ++dx) {
+            const int xx = clampi(x + dx, 0, W - 1);
+            const uint2 li = Lrow[xx];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int x2 = xx - (base + d0 + i);
+                sum[i] += (x2 >= 0 && x2 < W) ? btCost(li, Rrow[x2]) : 255;
+            }
+        }
+    }
+    int v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = ok[i] ? imin(sum[i], kMax - 1) : kMax;
+    store8(C + p * Dp + d0, v);
+}
+
+// ---- pyrDown (pyrdown.cu:16-52) ------------------------------------------------
+__device__ __forceinline__ int reflect101(int i, int n)
+{
+    if (i < 0) i = -i;
+    if (i >= n) i = 2 * n - 2 - i;
+    return i < 0 ? 0 : i;
+}
+
+__global__ __launch_bounds__(256) void kPyrDown(const uint8_t* srcA, int sstepA, const uint8_t* srcB,
+                                                int sstepB, int sw, int sh, uint8_t* dstA,
+                                                int dstepA, uint8_t* dstB, int dstepB, int dw,
+                                                int dh)
+{
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= dw || y >= dh) return;
+    const uint8_t* src = blockIdx.z ? srcB : srcA;
+    const int sstep = blockIdx.z ? sstepB : sstepA;
+    int cols[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) cols[k] = reflect101(2 * x + k - 2, sw);
+    constexpr int w[5] = {1, 4, 6, 4, 1};
+    int sum = 0;
+#pragma unroll
+    for (int ky = 0; ky < 5; ++ky) {
+        const uint8_t* row = src + (size_t)reflect101(2 * y + ky - 2, sh) * sstep;
+        const int rs = row[cols[0]] + 4 * row[cols[1]] + 6 * row[cols[2]] + 4 * row[cols[3]] +
+                       row[cols[4]];
+        sum += w[ky] * rs;
+    }
+    uint8_t* dst = blockIdx.z ? dstB : dstA;
+    const int dstep = blockIdx.z ? dstepB : dstepA;
+    dst[(size_t)y * dstep + x] = (uint8_t)((sum + 128) >> 8);
+}
+
+} // namespace
+
+hipError_t launchPrep(const LevelCall& c, Workspace& ws, hipStream_t s)
+{
+    dim3 grid((c.W + 63) / 64, (c.H + 3) / 4, 2);
+    kPrep<<<grid, 256, 0, s>>>(c.left, c.lstep, c.right, c.rstep, ws.desc, ws.keys, c.W, c.H,
+                               c.cost == 1, c.cap);
+    return hipGetLastError();
+}
+
+hipError_t launchCost(const LevelCall& c, Workspace& ws, hipStream_t s)
+{
+    const int tpp = c.Dp / 8;
+    const int npx = 256 / tpp;
+    const int half = c.block / 2;
+    if (c.mode == kRefine) {
+        RefineIn r{c.pdisp, c.pconf, c.pdpitch, c.pcpitch, c.PW, c.PH, c.minConf};
+        dim3 grid((c.W + npx - 1) / npx, c.H);
+        if (c.cost == 1)
+            kCensusLocal<<<grid, 256, 0, s>>>(ws.desc, r, ws.labels, ws.C, c.W, c.H, c.D, c.Dp,
+                                              c.minD, c.maxD);
+        else
+            kBTLocal<<<grid, 256, 0, s>>>(ws.desc, r, ws.labels, ws.C, c.W, c.H, c.D, c.Dp,
+                                          c.minD, c.maxD, half);
+        return hipGetLastError();
+    }
+    if (c.cost == 1) {
+        dim3 grid((c.W + npx - 1) / npx, c.H);
+        const size_t lds = (size_t)(npx + c.Dp - 1) * sizeof(uint32_t);
+        kCensusCost<<<grid, 256, lds, s>>>(ws.desc, ws.C, c.W, c.H, c.D, c.Dp, c.minD);
+        return hipGetLastError();
+    }
+    const int dc = c.Dp < kChunkD ? c.Dp : kChunkD;
+    dim3 gridH((c.W + kSegX - 1) / kSegX, c.H, c.Dp / dc);
+    const size_t lds = (size_t)(kSegX + 2 * half) * dc;
+    kBTRow<<<gridH, 256, lds, s>>>(ws.desc, ws.Alr, c.W, c.H, c.Dp, c.minD, half);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const size_t nOct = (size_t)c.W * c.H * (c.Dp / 8);
+    kBoxV<<<(unsigned)((nOct + 255) / 256), 256, 0, s>>>(ws.Alr, ws.C, c.W, c.H, c.Dp, half);
+    return hipGetLastError();
+}
+
+hipError_t launchPyrDown2(const uint8_t* srcA, int sstepA, const uint8_t* srcB, int sstepB,
+                          int sw, int sh, uint8_t* dstA, int dstepA, uint8_t* dstB, int dstepB,
+                          int dw, int dh, hipStream_t stream)
+{
+    dim3 grid((dw + 63) / 64, (dh + 3) / 4, srcB ? 2 : 1);
+    kPyrDown<<<grid, 256, 0, stream>>>(srcA, sstepA, srcB, sstepB, sw, sh, dstA, dstepA, dstB,
+                                       dstepB, dw, dh);
+    return hipGetLastError();
+}
+
+hipError_t launchPyrDown(const uint8_t* src, int sstep, int sw, int sh, uint8_t* dst, int dstep,
+                         int dw, int dh, hipStream_t stream)
+{
+    return launchPyrDown2(src, sstep, nullptr, 0, sw, sh, dst, dstep, nullptr, 0, dw, dh, stream);
+}
+
+} // namespace pyr

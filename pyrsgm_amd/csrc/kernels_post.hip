@@ -1,0 +1,174 @@
+// kernels_post.hip -- left-right check, 3x3 median, confidence clear and the
+// output conversions of one level, fused into one pass over the level.
+//
+// The right-view disparity of column x2 is decoded from the projection key
+// written by the R->L kernel: key = ((cost + 32768) << 16) | (0xFFFF - x) of the
+// winning left pixel x (smallest cost, ties to the largest x -- the order in
+// which the reference's sequential R->L update keeps a value,
+// sgm_kernels.cu:794-797 / :1462-1465), so disp2(x2) = x - x2.
+//
+// Restated semantics: pseudo LR check sgm_kernels.cu:825-851 (single scale),
+// strict LR check :855-889 (pyramid), median :891-920 (in-bounds neighbours,
+// sorted[n/2]), confidence clear :922-931, float conversion :1530-1551, output
+// routing :1837-1858 / :2053-2069.
+#include "device_common.h"
+#include "internal.h"
+
+namespace pyr {
+namespace {
+
+constexpr int kTile = 16;
+
+struct PostArgs {
+    const int16_t* disp;
+    const float* conf;
+    const uint32_t* keys;
+    int W, H, minD, maxDiff;
+    int lrcheck, median, strict, pyramid;
+    int16_t* out16;
+    float* out32;
+    float* outc;
+    size_t s16, s32, sc;  // bytes
+};
+
+__device__ __forceinline__ int disp2At(const PostArgs& a, int y, int x2, int invalid)
+{
+    const uint32_t k = a.keys[(size_t)y * a.W + x2];
+    if (k == 0xFFFFFFFFu) return invalid;
+    const int xw = 0xFFFF - (int)(k & 0xFFFFu);
+    return (int)(int16_t)(xw - x2);
+}
+
+__device__ __forceinline__ int checked(const PostArgs& a, int x, int y)
+{
+    const int invalid = (int)(int16_t)((a.minD - 1) * 16);
+    const int d = a.disp[(size_t)y * a.W + x];
+    if (!a.lrcheck || d == invalid) return d;
+    const int d0 = d >> 4, d1 = (d + 15) >> 4;
+    const int xa = x - d0, xb = x - d1;
+    const bool ina = xa >= 0 && xa < a.W, inb = xb >= 0 && xb < a.W;
+    const int ra = ina ? disp2At(a, y, xa, invalid) : 0;
+    const int rb = inb ? disp2At(a, y, xb, invalid) : 0;
+    if (a.strict) {
+        const bool oka = ina && ra >= a.minD && abs(ra - d0) <= a.maxDiff;
+        const bool okb = inb && rb >= a.minD && abs(rb - d1) <= a.maxDiff;
+        return (oka || okb) ? d : invalid;
+    }
+    const bool bada = ina && ra >= a.minD && abs(ra - d0) > a.maxDiff;
+    const bool badb = inb && rb >= a.minD && abs(rb - d1) > a.maxDiff;
+    return (bada && badb) ? invalid : d;
+}
+
+__device__ __forceinline__ void sort2(int& a, int& b)
+{
+    const int lo = imin(a, b), hi = imax(a, b);
+    a = lo;
+    b = hi;
+}
+
+// Median of 9 with a 19-exchange network.  Out-of-image neighbours are replaced
+// by 4 - n/2 copies of -inf and the rest by +inf, so that element 4 of the
+// sorted nine equals element n/2 of the n in-image values.
+__device__ __forceinline__ int median9(int p[9])
+{
+    sort2(p[1], p[2]); sort2(p[4], p[5]); sort2(p[7], p[8]);
+    sort2(p[0], p[1]); sort2(p[3], p[4]); sort2(p[6], p[7]);
+    sort2(p[1], p[2]); sort2(p[4], p[5]); sort2(p[7], p[8]);
+    sort2(p[0], p[3]); sort2(p[5], p[8]); sort2(p[4], p[7]);
+    sort2(p[3], p[6]); sort2(p[1], p[4]); sort2(p[2], p[5]);
+    sort2(p[4], p[7]); sort2(p[4], p[2]); sort2(p[6], p[4]);
+    sort2(p[4], p[2]);
+    return p[4];
+}
+
+__global__ __launch_bounds__(256) void kPost(PostArgs a)
+{
+    __shared__ int16_t s_d[kTile + 2][kTile + 2];
+    const int tx = threadIdx.x & (kTile - 1), ty = threadIdx.x / kTile;
+    const int bx = blockIdx.x * kTile, by = blockIdx.y * kTile;
+    if (a.median) {
+        for (int i = threadIdx.x; i < (kTile + 2) * (kTile + 2); i += 256) {
+            const int ly = i / (kTile + 2), lx = i - ly * (kTile + 2);
+            const int gx = bx + lx - 1, gy = by + ly - 1;
+            if (gx >= 0 && gx < a.W && gy >= 0 && gy < a.H) s_d[ly][lx] = (int16_t)checked(a, gx, gy);
+        }
+        __syncthreads();
+    }
+    const int x = bx + tx, y = by + ty;
+    if (x >= a.W || y >= a.H) return;
+    const size_t p = (size_t)y * a.W + x;
+    const int invalid = (int)(int16_t)((a.minD - 1) * 16);
+    const int raw = a.disp[p];
+    const int chk = a.median ? s_d[ty + 1][tx + 1] : checked(a, x, y);
+    float cf = (chk == raw) ? a.conf[p] : 0.0f;
+    int fin = chk;
+    if (a.median) {
+        int v[9];
+        int n = 0;
+#pragma unroll
+        for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+            for (int dx = -1; dx <= 1; ++dx) {
+                const bool in = x + dx >= 0 && x + dx < a.W && y + dy >= 0 && y + dy < a.H;
+                n += in;
+            }
+        const int nLow = 4 - n / 2;
+        int seenOut = 0;
+        int idx = 0;
+#pragma unroll
+        for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+            for (int dx = -1; dx <= 1; ++dx) {
+                const bool in = x + dx >= 0 && x + dx < a.W && y + dy >= 0 && y + dy < a.H;
+                int val;
+                if (in) {
+                    val = s_d[ty + 1 + dy][tx + 1 + dx];
+                } else {
+                    val = seenOut < nLow ? -100000 : 100000;
+                    ++seenOut;
+                }
+                v[idx++] = val;
+            }
+        fin = median9(v);
+        if (fin != chk) cf = 0.0f;
+    }
+    if (a.out16) *reinterpret_cast<int16_t*>((char*)a.out16 + (size_t)y * a.s16 + (size_t)x * 2) = (int16_t)fin;
+    if (a.out32 && a.outc) {
+        const float d = (float)fin / 16.0f;
+        const bool ok = fin != invalid && d > 0.0f;
+        *reinterpret_cast<float*>((char*)a.out32 + (size_t)y * a.s32 + (size_t)x * 4) = ok ? d : 0.0f;
+        *reinterpret_cast<float*>((char*)a.outc + (size_t)y * a.sc + (size_t)x * 4) = ok ? cf : 0.0f;
+    } else if (a.outc && a.pyramid) {
+        *reinterpret_cast<float*>((char*)a.outc + (size_t)y * a.sc + (size_t)x * 4) = cf;
+    }
+}
+
+} // namespace
+
+hipError_t launchPost(const LevelCall& c, Workspace& ws, hipStream_t s)
+{
+    PostArgs a;
+    a.disp = ws.disp;
+    a.conf = ws.conf;
+    a.keys = ws.keys;
+    a.W = c.W;
+    a.H = c.H;
+    a.minD = c.minD;
+    a.maxDiff = c.disp12;
+    a.lrcheck = (c.post & 1) != 0;
+    a.median = (c.post & 2) != 0;
+    a.strict = c.mode != kSingle;
+    a.pyramid = c.mode != kSingle;
+    a.out16 = c.out16;
+    a.out32 = c.out32;
+    a.outc = c.outc;
+    a.s16 = c.s16;
+    a.s32 = c.s32;
+    a.sc = c.sc;
+    if (!a.out16 && !a.outc) return hipSuccess;
+    dim3 grid((c.W + kTile - 1) / kTile, (c.H + kTile - 1) / kTile);
+    kPost<<<grid, 256, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+} // namespace pyr

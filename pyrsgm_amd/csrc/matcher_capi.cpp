@@ -1,0 +1,160 @@
+// matcher_capi.cpp -- plain-C handles on pyrsgm::StereoMatcher (pyrsgm_py.h).
+#include <stdio.h>
+#include <string.h>
+
+#include <exception>
+#include <new>
+
+#include "pyrsgm/pyrsgm.h"
+#include "pyrsgm/pyrsgm_py.h"
+
+namespace {
+
+void copyErr(const char* what, char* err, int errlen)
+{
+    if (err && errlen > 0) snprintf(err, (size_t)errlen, "%s", what);
+}
+
+pyrsgm::Params toParams(const pyrsgm_params_c& c)
+{
+    pyrsgm::Params p;
+    p.num_disparities = c.num_disparities;
+    p.min_disparity = c.min_disparity;
+    p.block_size = c.block_size;
+    p.cost_mode = c.cost_mode ? pyrsgm::CostMode::Census : pyrsgm::CostMode::BirchfieldTomasi;
+    p.p1 = c.p1;
+    p.p2 = c.p2;
+    p.uniqueness_ratio = c.uniqueness_ratio;
+    p.disp12_max_diff = c.disp12_max_diff;
+    p.lr_check = c.lr_check != 0;
+    p.median_filter = c.median_filter != 0;
+    p.num_paths = c.num_paths;
+    p.pre_filter_cap = c.pre_filter_cap;
+    p.pyramid_levels = c.pyramid_levels;
+    p.pyramid_coarse_disparities = c.pyramid_coarse_disparities;
+    p.pyramid_max_disparity = c.pyramid_max_disparity;
+    p.pyramid_min_confidence = c.pyramid_min_confidence;
+    p.adaptive_p2 = c.adaptive_p2 != 0;
+    p.p2_alpha = c.p2_alpha;
+    p.p2_beta = c.p2_beta;
+    p.p2_lower_mult = c.p2_lower_mult;
+    p.p2_upper_mult = c.p2_upper_mult;
+    return p;
+}
+
+pyrsgm_params_c fromParams(const pyrsgm::Params& p)
+{
+    pyrsgm_params_c c;
+    c.num_disparities = p.num_disparities;
+    c.min_disparity = p.min_disparity;
+    c.block_size = p.block_size;
+    c.cost_mode = p.cost_mode == pyrsgm::CostMode::Census ? 1 : 0;
+    c.p1 = p.p1;
+    c.p2 = p.p2;
+    c.uniqueness_ratio = p.uniqueness_ratio;
+    c.disp12_max_diff = p.disp12_max_diff;
+    c.lr_check = p.lr_check ? 1 : 0;
+    c.median_filter = p.median_filter ? 1 : 0;
+    c.num_paths = p.num_paths;
+    c.pre_filter_cap = p.pre_filter_cap;
+    c.pyramid_levels = p.pyramid_levels;
+    c.pyramid_coarse_disparities = p.pyramid_coarse_disparities;
+    c.pyramid_max_disparity = p.pyramid_max_disparity;
+    c.pyramid_min_confidence = p.pyramid_min_confidence;
+    c.adaptive_p2 = p.adaptive_p2 ? 1 : 0;
+    c.p2_alpha = p.p2_alpha;
+    c.p2_beta = p.p2_beta;
+    c.p2_lower_mult = p.p2_lower_mult;
+    c.p2_upper_mult = p.p2_upper_mult;
+    return c;
+}
+
+} // namespace
+
+extern "C" {
+
+int pyrsgm_validate_params(const pyrsgm_params_c* in, pyrsgm_params_c* out, char* err, int errlen)
+{
+    if (!in || !out) return -1;
+    try {
+        *out = fromParams(pyrsgm::validateParams(toParams(*in)));
+        return 0;
+    } catch (const std::exception& e) {
+        copyErr(e.what(), err, errlen);
+        return -1;
+    }
+}
+
+void* pyrsgm_matcher_create(int width, int height, const pyrsgm_params_c* params, char* err,
+                            int errlen)
+{
+    if (!params) return nullptr;
+    try {
+        return new pyrsgm::StereoMatcher(width, height, toParams(*params));
+    } catch (const std::exception& e) {
+        copyErr(e.what(), err, errlen);
+        return nullptr;
+    }
+}
+
+void pyrsgm_matcher_destroy(void* matcher)
+{
+    delete static_cast<pyrsgm::StereoMatcher*>(matcher);
+}
+
+int pyrsgm_matcher_compute(void* matcher, const uint8_t* d_left, size_t left_pitch,
+                           const uint8_t* d_right, size_t right_pitch, int16_t* d_disp16,
+                           size_t disp16_pitch, float* d_disp32, size_t disp32_pitch,
+                           float* d_conf, size_t conf_pitch, hipStream_t stream, char* err,
+                           int errlen)
+{
+    if (!matcher) return -1;
+    try {
+        static_cast<pyrsgm::StereoMatcher*>(matcher)->compute(
+            d_left, left_pitch, d_right, right_pitch, d_disp16, disp16_pitch, d_disp32,
+            disp32_pitch, d_conf, conf_pitch, stream);
+        return 0;
+    } catch (const std::exception& e) {
+        copyErr(e.what(), err, errlen);
+        return -1;
+    }
+}
+
+int pyrsgm_matcher_level_info(void* matcher, int level, int out[7])
+{
+    if (!matcher || !out) return -1;
+    try {
+        const pyrsgm::LevelInfo li = static_cast<pyrsgm::StereoMatcher*>(matcher)->levelInfo(level);
+        out[0] = li.level;
+        out[1] = li.width;
+        out[2] = li.height;
+        out[3] = li.num_labels;
+        out[4] = li.max_disparity;
+        out[5] = li.block_size;
+        out[6] = li.refine ? 1 : 0;
+        return 0;
+    } catch (const std::exception&) {
+        return -1;
+    }
+}
+
+int pyrsgm_matcher_num_levels(void* matcher)
+{
+    return matcher ? static_cast<pyrsgm::StereoMatcher*>(matcher)->numLevels() : -1;
+}
+
+int pyrsgm_matcher_params(void* matcher, pyrsgm_params_c* out)
+{
+    if (!matcher || !out) return -1;
+    *out = fromParams(static_cast<pyrsgm::StereoMatcher*>(matcher)->params());
+    return 0;
+}
+
+size_t pyrsgm_matcher_device_bytes(void* matcher)
+{
+    return matcher ? static_cast<pyrsgm::StereoMatcher*>(matcher)->deviceMemoryBytes() : 0;
+}
+
+const char* pyrsgm_version(void) { return pyrsgm::version(); }
+
+} // extern "C"

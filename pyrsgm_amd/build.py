@@ -26,6 +26,7 @@ SOURCES = [
     "capi.cpp",
     "stereo_matcher.cpp",
     "matcher_capi.cpp",
+    "prof.cpp",
 ]
 
 ARCH = os.environ.get("PYRSGM_OFFLOAD_ARCH", "gfx950")
@@ -50,8 +51,19 @@ def common_flags() -> list[str]:
     ]
 
 
-def _compile(src: str) -> str:
+def _deps_mtime() -> float:
+    hdrs = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".h")]
+    inc = os.path.join(ROOT, "include", "pyrsgm")
+    hdrs += [os.path.join(inc, f) for f in os.listdir(inc) if f.endswith(".h")]
+    return max(os.path.getmtime(h) for h in hdrs)
+
+
+def _compile(src: str, force: bool = True) -> str:
     obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
+    if not force and os.path.exists(obj):
+        newest = max(os.path.getmtime(os.path.join(CSRC, src)), _deps_mtime())
+        if os.path.getmtime(obj) > newest:
+            return obj
     cmd = [hipcc(), *common_flags(), "-c", os.path.join(CSRC, src), "-o", obj]
     if src.endswith(".cpp"):
         cmd.insert(1, "-x")
@@ -66,11 +78,12 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(jobs: int | None = None) -> str:
+def build(jobs: int | None = None, force: bool = True) -> str:
+    """Compile every source (force) or only out-of-date ones, then link."""
     os.makedirs(OBJDIR, exist_ok=True)
     jobs = jobs or min(len(SOURCES), max(1, (os.cpu_count() or 4)))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+        objs = list(ex.map(lambda s: _compile(s, force), SOURCES))
     tmp = LIB + ".tmp"
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -81,4 +94,4 @@ def build(jobs: int | None = None) -> str:
 
 
 if __name__ == "__main__":
-    print(build())
+    print(build(force="--incremental" not in sys.argv))

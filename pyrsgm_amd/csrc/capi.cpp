@@ -17,6 +17,7 @@
 
 #include "internal.h"
 #include "pyrsgm/pyrsgm_c.h"
+#include "pyrsgm/pyrsgm_prof.h"
 
 namespace pyr {
 
@@ -115,9 +116,13 @@ int buildP2Lut(int P2, int adaptive, float alpha, float beta, float lower, float
 
 hipError_t launchLevel(const LevelCall& c, Workspace& ws, hipStream_t s)
 {
+    profBegin(PYRSGM_STAGE_PREP, c.W, c.H, c.D, s);
     hipError_t e = launchPrep(c, ws, s);
+    profEnd(s);
     if (e != hipSuccess) return e;
+    profBegin(PYRSGM_STAGE_COST, c.W, c.H, c.D, s);
     e = launchCost(c, ws, s);
+    profEnd(s);
     if (e != hipSuccess) return e;
 
     PathArgs a;
@@ -146,11 +151,20 @@ hipError_t launchLevel(const LevelCall& c, Workspace& ws, hipStream_t s)
     a.minConf = c.minConf;
     memcpy(a.lut, c.lut, sizeof(a.lut));
 
-    e = launchPathsLRTB(a, c.mode, c.adaptive, c.numPaths >= 2, c.numPaths >= 3, s);
-    if (e != hipSuccess) return e;
+    if (c.numPaths >= 2) {
+        profBegin(PYRSGM_STAGE_LRTB, c.W, c.H, c.D, s);
+        e = launchPathsLRTB(a, c.mode, c.adaptive, 1, c.numPaths >= 3, s);
+        profEnd(s);
+        if (e != hipSuccess) return e;
+    }
+    profBegin(PYRSGM_STAGE_RL, c.W, c.H, c.D, s);
     e = launchPathRL(a, c.mode, c.adaptive, s);
+    profEnd(s);
     if (e != hipSuccess) return e;
-    return launchPost(c, ws, s);
+    profBegin(PYRSGM_STAGE_POST, c.W, c.H, c.D, s);
+    e = launchPost(c, ws, s);
+    profEnd(s);
+    return e;
 }
 
 } // namespace pyr

@@ -55,6 +55,10 @@ struct LevelCall {
 };
 
 hipError_t launchLevel(const LevelCall& c, Workspace& ws, hipStream_t stream);
+
+// Stage timing hook (pyrsgm_prof.h); no-ops unless a benchmark enabled it.
+void profBegin(int stage, int W, int H, int D, hipStream_t s);
+void profEnd(hipStream_t s);
 hipError_t launchPyrDown(const uint8_t* src, int sstep, int sw, int sh,
                          uint8_t* dst, int dstep, int dw, int dh, hipStream_t stream);
 hipError_t launchPyrDown2(const uint8_t* srcA, int sstepA, const uint8_t* srcB, int sstepB,

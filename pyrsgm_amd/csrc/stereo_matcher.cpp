@@ -15,6 +15,7 @@
 
 #include "internal.h"
 #include "pyrsgm/pyrsgm_c.h"
+#include "pyrsgm/pyrsgm_prof.h"
 
 namespace pyrsgm {
 namespace {
@@ -218,10 +219,12 @@ struct StereoMatcher::Impl {
         for (int level = 1; level <= levels; ++level) {
             const LevelInfo& parent = info[level - 1];
             const LevelInfo& li = info[level];
+            pyr::profBegin(PYRSGM_STAGE_PYRDOWN, li.width, li.height, 0, stream);
             const hipError_t e = pyr::launchPyrDown2(
                 pl, (int)plp, pr, (int)prp, parent.width, parent.height, pyrLeft[level].data,
                 (int)pyrLeft[level].pitch, pyrRight[level].data, (int)pyrRight[level].pitch,
                 li.width, li.height, stream);
+            pyr::profEnd(stream);
             if (e != hipSuccess) throwHip("pyrDown", e);
             pl = pyrLeft[level].data;
             plp = pyrLeft[level].pitch;

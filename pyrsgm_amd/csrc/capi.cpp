@@ -184,6 +184,13 @@ bool geometryOk(int W, int H, int D)
     return W > 0 && H > 0 && W <= kMaxWidth && D >= 16 && D <= 1024 && (D % 16) == 0;
 }
 
+// The kernels rely on non-negative penalties (the reference C++ API requires
+// 0 <= P1 <= P2); window sums are sized for blocks up to 255.
+bool paramsOk(int block, int P1, int P2)
+{
+    return block >= 1 && block <= 255 && P1 >= 0 && P2 >= 0;
+}
+
 } // namespace
 
 extern "C" {
@@ -218,7 +225,7 @@ int pyrsgm_compute_single_scale(
     float p2LowerMult, float p2UpperMult, hipStream_t stream)
 {
     if (!workspace || !d_left_gray || !d_right_gray || !geometryOk(width, height, numDisparities) ||
-        blockSize < 1 || blockSize > 255)
+        !paramsOk(blockSize, P1, P2))
         return -1;
     Workspace* ws = static_cast<Workspace*>(workspace);
     if (ensureWorkspace(ws, width, height, numDisparities) != 0) {
@@ -272,7 +279,7 @@ int pyrsgm_compute_level(
     float p2UpperMult, hipStream_t stream)
 {
     if (!workspace || !d_left_gray || !d_right_gray || !geometryOk(width, height, numDisparities) ||
-        (refineLevel && numDisparities > 256) || blockSize < 1 || blockSize > 255)
+        (refineLevel && numDisparities > 256) || !paramsOk(blockSize, P1, P2))
         return -1;
     if (refineLevel && (!d_parent_disp || !d_parent_conf || parent_width <= 0 || parent_height <= 0))
         return -1;

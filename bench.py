@@ -30,19 +30,21 @@ HBM_PEAK_GBS = 8000.0
 # 5070 Ti Laptop GPU with adaptive P2 off (README.md:91 of gegeges/pyrsgm).
 REF_FPS = 1000.0 / 15.66
 
-STAGES = {0: "prep", 1: "cost", 2: "paths_lr_tb", 3: "path_rl_wta", 4: "post", 5: "pyrdown"}
+STAGES = {0: "prep", 1: "cost", 2: "paths", 3: "wta", 4: "post", 5: "pyrdown"}
 
 
 def algorithmic_bytes(stage: str, w: int, h: int, labels: int) -> int:
     """Bytes a launch must move under the staged int16 dataflow convention:
     cost writes 2N; L->R 4N (read C, write A); T->B 6N (read C and A, write A);
-    R->L+WTA 4N + 12P; per-pixel maps are charged to the kernels that move them."""
+    R->L+WTA 4N + 12P.  This build's paths launch runs all three recurrences
+    (charged 4N + 6N + 4N) and its WTA launch carries the 12P of per-pixel
+    results; per-pixel maps of the other launches are charged to them."""
     P = w * h
     N = P * labels
     return {
         "cost": 2 * N,
-        "paths_lr_tb": 4 * N + 6 * N,
-        "path_rl_wta": 4 * N + 12 * P,
+        "paths": 4 * N + 6 * N + 4 * N,
+        "wta": 12 * P,
         "prep": 2 * P + 16 * P + 4 * P,
         "post": 10 * P + 10 * P,
         "pyrdown": 2 * P * 4 + 2 * P,

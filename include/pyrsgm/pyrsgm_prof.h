@@ -17,8 +17,8 @@ extern "C" {
 /* Stage ids reported by pyrsgm_prof_get. */
 #define PYRSGM_STAGE_PREP     0  /* census words / BT intervals + key reset */
 #define PYRSGM_STAGE_COST     1  /* matching-cost volume                     */
-#define PYRSGM_STAGE_LRTB     2  /* L->R and T->B paths (one launch)         */
-#define PYRSGM_STAGE_RL       3  /* R->L path + winner-take-all              */
+#define PYRSGM_STAGE_PATHS    2  /* L->R, R->L and T->B paths (one launch)   */
+#define PYRSGM_STAGE_WTA      3  /* path sums + winner-take-all              */
 #define PYRSGM_STAGE_POST     4  /* LR check + median + outputs              */
 #define PYRSGM_STAGE_PYRDOWN  5  /* pyramid down-sampling (both views)       */
 

@@ -20,7 +20,10 @@ OBJDIR = os.path.join(LIBDIR, "obj")
 LIB = os.path.join(LIBDIR, "libpyrsgm.so")
 
 SOURCES = [
-    "kernels_path.hip",
+    "kernels_path_refine.hip",
+    "kernels_path_coarse.hip",
+    "kernels_path_single.hip",
+    "kernels_wta.hip",
     "kernels_cost.hip",
     "kernels_post.hip",
     "capi.cpp",

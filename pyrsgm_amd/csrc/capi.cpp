@@ -4,8 +4,8 @@
 // Per level, five launches on the caller's stream and no host synchronisation:
 //   kPrep      census words / BT intervals of both views, projection keys reset
 //   cost       full-range census | BT + window sums | refinement-local costs
-//   kPathsLRTB L->R and T->B together
-//   kPathRL    R->L + winner-take-all
+//   kPaths     L->R, R->L and T->B path costs, all three in one launch
+//   kWTA       path sums + winner-take-all + uniqueness + sub-pixel + projection
 //   kPost      LR check + median + confidence clear + output conversions
 #include <math.h>
 #include <stdio.h>
@@ -26,7 +26,7 @@ constexpr size_t kAlign = 256;
 size_t alignUp(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
 struct Layout {
-    size_t desc, C, Alr, Atb, labels, disp, conf, keys, total;
+    size_t desc, C, Alr, Atb, Arl, labels, disp, conf, keys, total;
 };
 
 Layout layoutFor(int W, int H, int D)
@@ -39,6 +39,7 @@ Layout layoutFor(int W, int H, int D)
     l.C = o; o += alignUp(P * Dp * 2);
     l.Alr = o; o += alignUp(P * Dp * 2);
     l.Atb = o; o += alignUp(P * Dp * 2);
+    l.Arl = o; o += alignUp(P * Dp * 2);
     l.labels = o; o += alignUp(P * 4);
     l.disp = o; o += alignUp(P * 2);
     l.conf = o; o += alignUp(P * 4);
@@ -78,6 +79,7 @@ int ensureWorkspace(Workspace* ws, int W, int H, int D)
     ws->C = reinterpret_cast<int16_t*>(b + l.C);
     ws->Alr = reinterpret_cast<int16_t*>(b + l.Alr);
     ws->Atb = reinterpret_cast<int16_t*>(b + l.Atb);
+    ws->Arl = reinterpret_cast<int16_t*>(b + l.Arl);
     ws->labels = reinterpret_cast<int32_t*>(b + l.labels);
     ws->disp = reinterpret_cast<int16_t*>(b + l.disp);
     ws->conf = reinterpret_cast<float*>(b + l.conf);
@@ -114,6 +116,13 @@ int buildP2Lut(int P2, int adaptive, float alpha, float beta, float lower, float
     return adaptive;
 }
 
+hipError_t launchPaths(const PathArgs& a, int mode, int adaptive, hipStream_t s)
+{
+    if (mode == kRefine) return launchPathsRefine(a, adaptive, s);
+    if (mode == kCoarse) return launchPathsCoarse(a, adaptive, s);
+    return launchPathsSingle(a, adaptive, s);
+}
+
 hipError_t launchLevel(const LevelCall& c, Workspace& ws, hipStream_t s)
 {
     profBegin(PYRSGM_STAGE_PREP, c.W, c.H, c.D, s);
@@ -130,6 +139,7 @@ hipError_t launchLevel(const LevelCall& c, Workspace& ws, hipStream_t s)
     a.C = ws.C;
     a.Alr = ws.Alr;
     a.Atb = ws.Atb;
+    a.Arl = ws.Arl;
     a.labels = c.mode == kRefine ? ws.labels : nullptr;
     a.img = c.left;
     a.istep = c.lstep;
@@ -151,14 +161,12 @@ hipError_t launchLevel(const LevelCall& c, Workspace& ws, hipStream_t s)
     a.minConf = c.minConf;
     memcpy(a.lut, c.lut, sizeof(a.lut));
 
-    if (c.numPaths >= 2) {
-        profBegin(PYRSGM_STAGE_LRTB, c.W, c.H, c.D, s);
-        e = launchPathsLRTB(a, c.mode, c.adaptive, 1, c.numPaths >= 3, s);
-        profEnd(s);
-        if (e != hipSuccess) return e;
-    }
-    profBegin(PYRSGM_STAGE_RL, c.W, c.H, c.D, s);
-    e = launchPathRL(a, c.mode, c.adaptive, s);
+    profBegin(PYRSGM_STAGE_PATHS, c.W, c.H, c.D, s);
+    e = launchPaths(a, c.mode, c.adaptive, s);
+    profEnd(s);
+    if (e != hipSuccess) return e;
+    profBegin(PYRSGM_STAGE_WTA, c.W, c.H, c.D, s);
+    e = launchWTA(a, c.mode, s);
     profEnd(s);
     if (e != hipSuccess) return e;
     profBegin(PYRSGM_STAGE_POST, c.W, c.H, c.D, s);

@@ -29,6 +29,7 @@ struct Workspace {
     int16_t* C = nullptr;       // [P][Dp] matching cost
     int16_t* Alr = nullptr;     // [P][Dp] L->R path costs (also the BT row-sum scratch)
     int16_t* Atb = nullptr;     // [P][Dp] T->B path costs
+    int16_t* Arl = nullptr;     // [P][Dp] R->L path costs
     int32_t* labels = nullptr;  // [P] refinement label word: (base & 0xFFFF) | (valid << 16)
     int16_t* disp = nullptr;    // [P] raw disparity*16 after WTA
     float* conf = nullptr;      // [P] raw confidence after WTA
@@ -75,13 +76,15 @@ struct PathArgs {
     const int16_t* C;
     int16_t* Alr;
     int16_t* Atb;
+    int16_t* Arl;
     const int32_t* labels;     // refine only
     const uint8_t* img;        // level left image (adaptive P2)
     int istep;
     int W, H, D, Dp, minD, maxD, P1, P2;
     int startX, endX;          // single-scale L->R column range
     int numPaths;
-    int nLRBlocks;             // combined launch: blocks [0, nLRBlocks) run L->R
+    int nLRBlocks;             // combined launch: blocks [0, nLR) run L->R,
+    int nRLBlocks;             // [nLR, nLR+nRL) R->L, the rest T->B
     int16_t* disp;
     float* conf;
     uint32_t* keys;
@@ -90,9 +93,11 @@ struct PathArgs {
     int32_t lut[256];
 };
 
-hipError_t launchPathsLRTB(const PathArgs& a, int mode, int adaptive, int runLR, int runTB,
-                           hipStream_t s);
-hipError_t launchPathRL(const PathArgs& a, int mode, int adaptive, hipStream_t s);
+hipError_t launchPaths(const PathArgs& a, int mode, int adaptive, hipStream_t s);
+hipError_t launchPathsSingle(const PathArgs& a, int adaptive, hipStream_t s);
+hipError_t launchPathsCoarse(const PathArgs& a, int adaptive, hipStream_t s);
+hipError_t launchPathsRefine(const PathArgs& a, int adaptive, hipStream_t s);
+hipError_t launchWTA(const PathArgs& a, int mode, hipStream_t s);
 
 hipError_t launchPrep(const LevelCall& c, Workspace& ws, hipStream_t s);
 hipError_t launchCost(const LevelCall& c, Workspace& ws, hipStream_t s);

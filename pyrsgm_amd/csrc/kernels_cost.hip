@@ -103,8 +103,12 @@ __device__ __forceinline__ void store8(int16_t* dst, const int (&v)[8])
 // ---- Census cost over the full label range (sgm_kernels.cu:303-327) --------
 // Block: 256 threads = NPX pixels x (Dp/8) label octets of one row segment; the
 // right-view census words the segment can reach are staged in LDS.
+// At a coarse pyramid level (geomMask) a label whose real disparity is outside
+// [minD, min(maxD, x)] or whose right column is outside the image is stored as
+// SHRT_MAX: that is exactly the reference's labelValid = geometry && C < MAX
+// (sgm_kernels.cu:1135-1143), so the path kernels test "C < SHRT_MAX" alone.
 __global__ __launch_bounds__(256) void kCensusCost(const uint2* desc, int16_t* C, int W, int H,
-                                                   int D, int Dp, int minD)
+                                                   int D, int Dp, int minD, int maxD, int geomMask)
 {
     extern __shared__ uint32_t s_r[];
     const int tpp = Dp >> 3;
@@ -125,13 +129,15 @@ __global__ __launch_bounds__(256) void kCensusCost(const uint2* desc, int16_t* C
     const int x = x0 + px;
     if (x >= W) return;
     const uint32_t cl = L[x].x;
+    const int hiReal = imin(maxD, x);
     int v[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
         const int d = d0 + i;
         const int x2 = x - d - minD;
-        v[i] = d >= D ? kMax
-                      : ((x2 >= 0 && x2 < W) ? __popc(cl ^ s_r[x2 - base2]) : 24);
+        const bool in = x2 >= 0 && x2 < W;
+        v[i] = d >= D ? kMax : (in ? __popc(cl ^ s_r[x2 - base2]) : 24);
+        if (geomMask && (!in || minD + d > hiReal)) v[i] = kMax;
     }
     store8(C + ((size_t)y * W + x) * Dp + d0, v);
 }
@@ -243,8 +249,8 @@ __global__ __launch_bounds__(256) void kBTRow(const uint2* desc, int16_t* Hs, in
 }
 
 // ---- vertical window sum (sgm_kernels.cu:390-416) ----------------------------
-__global__ __launch_bounds__(256) void kBoxV(const int16_t* Hs, int16_t* C, int W, int H, int Dp,
-                                             int half)
+__global__ __launch_bounds__(256) void kBoxV(const int16_t* Hs, int16_t* C, int W, int H, int D,
+                                             int Dp, int half, int minD, int maxD, int geomMask)
 {
     const size_t oct = (size_t)blockIdx.x * 256 + threadIdx.x;
     const size_t nOct = (size_t)W * H * (Dp >> 3);
@@ -264,8 +270,14 @@ __global__ __launch_bounds__(256) void kBoxV(const int16_t* Hs, int16_t* C, int 
         v[4] += (int16_t)(w.z & 0xFFFF); v[5] += (int)w.z >> 16;
         v[6] += (int16_t)(w.w & 0xFFFF); v[7] += (int)w.w >> 16;
     }
+    const int hiReal = imin(maxD, x);
 #pragma unroll
-    for (int i = 0; i < 8; ++i) v[i] = imin(kMax, v[i]);
+    for (int i = 0; i < 8; ++i) {
+        v[i] = imin(kMax, v[i]);
+        const int real = minD + d0 + i;
+        // Coarse pyramid levels: geometry-invalid labels carry SHRT_MAX (see kCensusCost).
+        if (geomMask && (d0 + i >= D || real > hiReal || x - real < 0 || x - real >= W)) v[i] = kMax;
+    }
     store8(C + p * Dp + d0, v);
 }
 
@@ -380,7 +392,8 @@ hipError_t launchCost(const LevelCall& c, Workspace& ws, hipStream_t s)
     if (c.cost == 1) {
         dim3 grid((c.W + npx - 1) / npx, c.H);
         const size_t lds = (size_t)(npx + c.Dp - 1) * sizeof(uint32_t);
-        kCensusCost<<<grid, 256, lds, s>>>(ws.desc, ws.C, c.W, c.H, c.D, c.Dp, c.minD);
+        kCensusCost<<<grid, 256, lds, s>>>(ws.desc, ws.C, c.W, c.H, c.D, c.Dp, c.minD, c.maxD,
+                                           c.mode == kCoarse);
         return hipGetLastError();
     }
     const int dc = c.Dp < kChunkD ? c.Dp : kChunkD;
@@ -390,7 +403,8 @@ hipError_t launchCost(const LevelCall& c, Workspace& ws, hipStream_t s)
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const size_t nOct = (size_t)c.W * c.H * (c.Dp / 8);
-    kBoxV<<<(unsigned)((nOct + 255) / 256), 256, 0, s>>>(ws.Alr, ws.C, c.W, c.H, c.Dp, half);
+    kBoxV<<<(unsigned)((nOct + 255) / 256), 256, 0, s>>>(ws.Alr, ws.C, c.W, c.H, c.D, c.Dp, half,
+                                                          c.minD, c.maxD, c.mode == kCoarse);
     return hipGetLastError();
 }
 

@@ -1,0 +1,190 @@
+// kernels_wta.hip -- path sums + winner-take-all for one level.
+//
+// Q = Dp/16 lanes per pixel, 16 labels per lane, pixels consecutive across
+// lanes, so every volume read is a run of full 16-byte loads.  Per pixel:
+//   * sum the three path costs in the reference's order and saturation:
+//     single scale acc = min(MAX, Lr + Tb) then min(MAX, acc + Rl)
+//     (sgm_kernels.cu:634, :740-741); pyramid
+//     old = (Lr<MAX && Tb<MAX) ? min(MAX-1, Lr+Tb) : MAX, then the same with Rl
+//     (sgm_kernels.cu:1325-1328, :1407-1411); numPaths 2 and 1 drop T->B / L->R;
+//   * argmin over the packed key (cost << 16 | d): smallest cost, then smallest d
+//     (sgm_kernels.cu:750-753), reduced across the Q lanes with DPP;
+//   * second-best cost over |d - best| > 1, uniqueness, sub-pixel parabola,
+//     confidence, validity rules (single scale :759-808; pyramid :1422-1470);
+//   * the right-view projection disp2 as atomicMin of
+//     ((cost + 32768) << 16) | (0xFFFF - x): the smallest cost wins and, among
+//     equal costs, the largest x -- the pixel the reference's sequential
+//     right-to-left update keeps (sgm_kernels.cu:794-797, :1462-1465).
+#include "path_impl.h"
+
+namespace pyr {
+namespace {
+
+using path::gmin;
+using path::gumin;
+
+__device__ __forceinline__ void load16(int (&v)[16], const int16_t* p)
+{
+    const uint4 a = reinterpret_cast<const uint4*>(p)[0];
+    const uint4 b = reinterpret_cast<const uint4*>(p)[1];
+    const uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        v[2 * i] = (int)(w[i] & 0xFFFFu);
+        v[2 * i + 1] = (int)(w[i] >> 16);
+    }
+}
+
+template <int Q, int MODE>
+__global__ __launch_bounds__(256) void kWTA(PathArgs a)
+{
+    constexpr bool PYR = MODE != kSingle;
+    const int W = a.W, D = a.D;
+    const size_t P = (size_t)W * a.H;
+    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t p = t / Q;
+    const int sub = (int)(t - p * Q);
+    const bool ok = p < P;
+    const size_t pc = ok ? p : P - 1;
+    const size_t off = pc * a.Dp + (size_t)sub * 16;
+    const int np = a.numPaths;
+
+    int my[16];
+    {
+        int rl[16], lr[16], tb[16];
+        load16(rl, a.Arl + off);
+        if (np >= 2) load16(lr, a.Alr + off);
+        if (np >= 3) load16(tb, a.Atb + off);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            if constexpr (!PYR) {
+                int acc = np >= 2 ? lr[i] : 0;
+                if (np >= 3) acc = (int)(int16_t)imin(kMax, acc + tb[i]);
+                my[i] = (int)(int16_t)imin(kMax, acc + rl[i]);
+            } else {
+                int old = np >= 2 ? lr[i] : 0;
+                if (np >= 3) old = imax(lr[i], tb[i]) == kMax ? kMax : imin(kMax - 1, lr[i] + tb[i]);
+                my[i] = np >= 2 ? (imax(old, rl[i]) == kMax ? kMax : imin(kMax - 1, old + rl[i]))
+                                : rl[i];
+            }
+        }
+    }
+
+    const int d0 = sub * 16;
+    uint32_t kmin = 0xFFFFFFFFu;
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+        if (d0 + i < D) kmin = umin(kmin, ((uint32_t)(uint16_t)my[i] << 16) | (uint32_t)(d0 + i));
+    kmin = gumin<Q>(kmin);
+    const int minCost = (int)(int16_t)(kmin >> 16);
+    const int bestD = (int)(kmin & 0xFFFFu);
+    int alt = kMax, c0 = kMax, c1 = kMax, c2 = kMax;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int d = d0 + i;
+        if (d < D && abs(d - bestD) > 1) alt = imin(alt, my[i]);
+        if (d == bestD - 1) c0 = my[i];
+        if (d == bestD) c1 = my[i];
+        if (d == bestD + 1) c2 = my[i];
+    }
+    alt = gmin<Q>(alt);
+    if constexpr (Q > 1) {
+        c0 = gmin<Q>(c0);
+        c1 = gmin<Q>(c1);
+        c2 = gmin<Q>(c2);
+    }
+    if (sub != 0 || !ok) return;
+
+    const int y = (int)(p / W);
+    const int x = (int)(p - (size_t)y * W);
+    const int16_t invalid = (int16_t)((a.minD - 1) * 16);
+    int16_t out = invalid;
+    float cf = 0.0f;
+    if constexpr (!PYR) {
+        const int minX1 = imax(a.minD + D, 0);
+        const int maxX1 = W + imin(a.minD, 0);
+        // x = W-1 starts the R->L pass and is never matched (sgm_kernels.cu:710-716).
+        bool valid = x < W - 1 && x >= minX1 && x < maxX1;
+        if (a.uniq > 0 && minCost < kMax / 2 && alt * (100 - a.uniq) < minCost * 100) valid = false;
+        if (valid && minCost < kMax / 2) {
+            int dint = bestD * 16;
+            if (bestD > 0 && bestD < D - 1) {
+                const int den = imax(c0 + c2 - 2 * c1, 1);
+                dint += div_trunc((c0 - c2) * 16 + den, den * 2);
+            }
+            out = (int16_t)(dint + a.minD * 16);
+            const float denom = fmaxf((float)minCost, 1.0f);
+            cf = fminf(fmaxf(((float)alt - (float)minCost) / denom, 0.0f), 1.0f);
+            const int x2 = x - bestD - a.minD;
+            if (x2 >= 0 && x2 < W)
+                atomicMin(&a.keys[(size_t)y * W + x2],
+                          ((uint32_t)(minCost + 32768) << 16) | (uint32_t)(0xFFFF - x));
+        }
+    } else {
+        int base = a.minD, valid = 1;
+        if constexpr (MODE == kRefine) {
+            const int lw = a.labels[p];
+            base = (int)(int16_t)(lw & 0xFFFF);
+            valid = (lw >> 16) & 1;
+        }
+        const int lo = imax(a.minD, x - W + 1), hi = imin(a.maxD, x);
+        const int fl = imax(0, lo - base), ll = imin(D - 1, hi - base);
+        const int vc = (valid && ll >= fl) ? ll - fl + 1 : 0;
+        bool good = vc >= 3 && minCost < kMax && alt < kMax;
+        if (good && a.uniq > 0 && alt * (100 - a.uniq) < minCost * 100) good = false;
+        const int realBest = base + bestD;
+        const int physMax = imin(a.maxD, x);
+        if ((bestD == 0 && base > a.minD) || (bestD == D - 1 && base + D - 1 < physMax)) good = false;
+        float c = 0.0f;
+        if (good) {
+            const float denom = fmaxf((float)minCost, 1.0f);
+            c = fminf(fmaxf(((float)alt - (float)minCost) / denom, 0.0f), 1.0f);
+            if (c < a.minConf) good = false;
+        }
+        if (good) {
+            int d16 = realBest * 16;
+            if (bestD > 0 && bestD + 1 < D && c0 < kMax && c1 < kMax && c2 < kMax) {
+                const int den = imax(c0 + c2 - 2 * c1, 1);
+                d16 += div_trunc((c0 - c2) * 16 + den, den * 2);
+            }
+            out = (int16_t)d16;
+            cf = c;
+            const int x2 = x - realBest;
+            if (x2 >= 0 && x2 < W)
+                atomicMin(&a.keys[(size_t)y * W + x2],
+                          ((uint32_t)(minCost + 32768) << 16) | (uint32_t)(0xFFFF - x));
+        }
+    }
+    a.disp[p] = out;
+    a.conf[p] = cf;
+}
+
+template <int MODE>
+hipError_t dispatchWTA(const PathArgs& a, hipStream_t s)
+{
+    const int Q = a.Dp / 16;
+    const size_t threads = (size_t)a.W * a.H * Q;
+    const unsigned blocks = (unsigned)((threads + 255) / 256);
+    switch (Q) {
+    case 1: kWTA<1, MODE><<<blocks, 256, 0, s>>>(a); break;
+    case 2: kWTA<2, MODE><<<blocks, 256, 0, s>>>(a); break;
+    case 4: kWTA<4, MODE><<<blocks, 256, 0, s>>>(a); break;
+    case 8: kWTA<8, MODE><<<blocks, 256, 0, s>>>(a); break;
+    case 16: kWTA<16, MODE><<<blocks, 256, 0, s>>>(a); break;
+    case 32: kWTA<32, MODE><<<blocks, 256, 0, s>>>(a); break;
+    case 64: kWTA<64, MODE><<<blocks, 256, 0, s>>>(a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+} // namespace
+
+hipError_t launchWTA(const PathArgs& a, int mode, hipStream_t s)
+{
+    if (mode == kRefine) return dispatchWTA<kRefine>(a, s);
+    if (mode == kCoarse) return dispatchWTA<kCoarse>(a, s);
+    return dispatchWTA<kSingle>(a, s);
+}
+
+} // namespace pyr

@@ -24,7 +24,7 @@ import os
 from dataclasses import dataclass, fields
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libpyrsgm.so")
+LIB_PATH = os.environ.get("PYRSGM_LIB", os.path.join(HERE, "lib", "libpyrsgm.so"))
 
 PYRSGM_COST_BT = 0
 PYRSGM_COST_CENSUS = 1

@@ -15,7 +15,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
-LIBDIR = os.path.join(HERE, "lib")
+LIBDIR = os.environ.get("PYRSGM_LIBDIR", os.path.join(HERE, "lib"))
 OBJDIR = os.path.join(LIBDIR, "obj")
 LIB = os.path.join(LIBDIR, "libpyrsgm.so")
 
@@ -43,7 +43,8 @@ def hipcc() -> str:
 
 
 def common_flags() -> list[str]:
-    return [
+    extra = os.environ.get("PYRSGM_EXTRA_FLAGS", "").split()
+    return extra + [
         "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
         # Float expressions evaluated operation by operation (confidence, P2 table).
         "-ffp-contract=off",

@@ -116,6 +116,26 @@ int buildP2Lut(int P2, int adaptive, float alpha, float beta, float lower, float
     return adaptive;
 }
 
+// The pyramid path kernels may drop the reference's SHRT_MAX tests when an
+// active line's jump term minPrev + P2' provably stays below SHRT_MAX:
+// minPrev <= Cmax + max(P2') (a path cost never exceeds its pixel's cost plus
+// the largest penalty), so Cmax + 2 * max(P2') < SHRT_MAX suffices.
+int exactPathsNeeded(const LevelCall& c)
+{
+    if (c.mode == kSingle) return 0;
+    long cmax;
+    if (c.cost == 1) {
+        cmax = 24;
+    } else {
+        cmax = 255L * c.block * c.block;
+        if (cmax > 32766) cmax = 32766;
+    }
+    long p2max = 0;
+    for (int i = 0; i < 256; ++i) p2max = std::max<long>(p2max, c.lut[i]);
+    p2max = std::max<long>(p2max, c.P2);
+    return cmax + 2 * p2max >= 32767 ? 1 : 0;
+}
+
 hipError_t launchPaths(const PathArgs& a, int mode, int adaptive, hipStream_t s)
 {
     if (mode == kRefine) return launchPathsRefine(a, adaptive, s);
@@ -154,6 +174,7 @@ hipError_t launchLevel(const LevelCall& c, Workspace& ws, hipStream_t s)
     a.startX = std::max(c.minD + c.D, 0);
     a.endX = c.W + std::min(c.minD, 0);
     a.numPaths = c.numPaths;
+    a.exact = exactPathsNeeded(c);
     a.disp = ws.disp;
     a.conf = ws.conf;
     a.keys = ws.keys;

@@ -83,6 +83,7 @@ struct PathArgs {
     int W, H, D, Dp, minD, maxD, P1, P2;
     int startX, endX;          // single-scale L->R column range
     int numPaths;
+    int exact;                 // pyramid: Cmax + 2*max(P2') may reach SHRT_MAX
     int nLRBlocks;             // combined launch: blocks [0, nLR) run L->R,
     int nRLBlocks;             // [nLR, nLR+nRL) R->L, the rest T->B
     int16_t* disp;

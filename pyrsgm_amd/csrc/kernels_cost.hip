@@ -275,8 +275,9 @@ __global__ __launch_bounds__(256) void kBoxV(const int16_t* Hs, int16_t* C, int 
     for (int i = 0; i < 8; ++i) {
         v[i] = imin(kMax, v[i]);
         const int real = minD + d0 + i;
-        // Coarse pyramid levels: geometry-invalid labels carry SHRT_MAX (see kCensusCost).
-        if (geomMask && (d0 + i >= D || real > hiReal || x - real < 0 || x - real >= W)) v[i] = kMax;
+        // Padding labels, and at coarse pyramid levels geometry-invalid labels,
+        // carry SHRT_MAX (see kCensusCost).
+        if (d0 + i >= D || (geomMask && (real > hiReal || x - real < 0 || x - real >= W))) v[i] = kMax;
     }
     store8(C + p * Dp + d0, v);
 }

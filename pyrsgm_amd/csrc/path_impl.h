@@ -42,9 +42,13 @@ enum Dir : int { kLR = 0, kTB = 1, kRL = 2 };
 constexpr uint32_t kRsrcWord3 = 0x00020000u;
 constexpr uint32_t kOffLimit = 0x80000000u;  // offset given to lanes of absent lines
 
-// Pixel steps fetched ahead: about 32 labels per lane in flight.
+// Pixel steps fetched ahead (compile-time tuning knob): about 32 labels per
+// lane in flight by default.
+#ifndef PYRSGM_PREFETCH_STEPS
+#define PYRSGM_PREFETCH_STEPS 16
+#endif
 template <int K>
-constexpr int prefetchSteps() { return K <= 2 ? 16 : 32 / K; }
+constexpr int prefetchSteps() { return K <= 2 ? PYRSGM_PREFETCH_STEPS : 32 / K; }
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* base, uint32_t bytes)
 {
@@ -174,8 +178,10 @@ struct ChunkAddr {
     uint32_t nr;      // bytes reachable from origin by the wave's present lines
 };
 
-// MODE: kSingle, kCoarse or kRefine.
-template <int G, int K, int MODE, bool ADAPT, int DIR>
+// MODE: kSingle, kCoarse or kRefine.  EXACT (pyramid only): keep the
+// reference's explicit "no active path" flag and SHRT_MAX tests, needed when
+// Cmax + 2 * max(P2') could reach SHRT_MAX.
+template <int G, int K, int MODE, bool ADAPT, bool EXACT, int DIR>
 __device__ __forceinline__ void runLine(const PathArgs& a, int lineGroup, const int* s_lut,
                                         int* s_xchg)
 {
@@ -251,63 +257,82 @@ __device__ __forceinline__ void runLine(const PathArgs& a, int lineGroup, const 
         }
     };
 
-    // Path state of this lane's line (see the header for the inactive-line convention).
+    // Path state of this lane's line.  Fast form (EXACT == false): a line
+    // without an active path holds SHRT_MAX everywhere, so the next step's
+    // best is SHRT_MAX and min(MAX, C + best - minPrev) = C.  Exact form: a
+    // line without an active path holds zeros and an explicit flag.
     int Lp[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) Lp[k] = 0;
-    int minPrev = 0;
+    for (int k = 0; k < K; ++k) Lp[k] = EXACT ? 0 : kMax;
+    int minPrev = EXACT ? 0 : kMax;
     int prevBase = a.minD;
     bool active = false;
     int prevIm = 0;
     const int P1 = a.P1;
 
-    auto step = [&](const StepIn<K>& st, int p2, __amdgpu_buffer_rsrc_t rOut, uint32_t soffOut) {
+    auto step = [&](const StepIn<K>& st, int p2, bool moved, __amdgpu_buffer_rsrc_t rOut,
+                    uint32_t soffOut) {
         int c[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) c[k] = unpk<K>(st.c, k);
-        const int lj = imin(kMax, minPrev + p2);
         int Lc[K];
         if constexpr (!PYR) {
+            // L = min(MAX, C + min(m3, min(MAX, minPrev + P2')) - minPrev) with
+            // m3 = min(L[d], L[d-1]+P1, L[d+1]+P1) >= minPrev, split into
+            // min3(C + m3 - minPrev, C + min(MAX - minPrev, P2'), MAX): no select,
+            // and padding labels (C = SHRT_MAX) stay SHRT_MAX.
             const int lmS = from_lower<G>(Lp[K - 1], kMax, g);
             const int lpS = from_upper<G>(Lp[0], kMax, g);
+            const int cm = 0 - minPrev;
+            const int q = imin(kMax - minPrev, p2);
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 const int Lm = k > 0 ? Lp[k - 1] : lmS;
                 const int Lq = k < K - 1 ? Lp[k + 1] : lpS;
-                const int best = imin(imin(Lm + P1, Lq + P1), imin(Lp[k], lj));
-                Lc[k] = sat16(c[k] + best - minPrev);
-                if (GK > 32 && g * K + k >= D) Lc[k] = kMax;  // padding labels
+                const int m3 = imin(imin(Lm + P1, Lq + P1), Lp[k]);
+                Lc[k] = imin(imin(c[k] + cm + m3, c[k] + q), kMax);
             }
         } else {
-            int base = a.minD;
-            if constexpr (REFINE) base = (int)(int16_t)(st.lw & 0xFFFF);
             int l0[K], lm[K], lp[K];
-            bool moved = false;
             if constexpr (REFINE) {
+                const int base = (int)(int16_t)(st.lw & 0xFFFF);
                 const int sh = base - prevBase;
-                moved = __ballot(active && sh != 0) != 0;
-                if (moved) {
-                    // A line's label window moved: re-index its previous costs through LDS.
-                    const int she = active ? sh : 0;
-                    __syncthreads();
+                prevBase = base;
+                if (__builtin_expect(moved, 0)) {
+                    // Some line's label window moved: re-index its previous costs.
+                    const int she = (!EXACT || active) ? sh : 0;
+                    if constexpr (K == 1) {
+                        const int sm = g + she;
+                        const int lane0 = j * G;
+                        const int i0 = lane0 + imin(imax(sm, 0), D - 1);
+                        const int im = lane0 + imin(imax(sm - 1, 0), D - 1);
+                        const int ip = lane0 + imin(imax(sm + 1, 0), D - 1);
+                        const int v0 = __builtin_amdgcn_ds_bpermute(i0 << 2, Lp[0]);
+                        const int vm = __builtin_amdgcn_ds_bpermute(im << 2, Lp[0]);
+                        const int vp = __builtin_amdgcn_ds_bpermute(ip << 2, Lp[0]);
+                        l0[0] = (sm >= 0 && sm < D) ? v0 : kMax;
+                        lm[0] = ((sm >= 1 && sm <= D) ? vm : kMax) + P1;
+                        lp[0] = ((sm >= -1 && sm < D - 1) ? vp : kMax) + P1;
+                    } else {
+                        __syncthreads();
 #pragma unroll
-                    for (int k = 0; k < K; ++k) s_xchg[lane * K + k] = Lp[k];
-                    __syncthreads();
-                    const int* lineX = s_xchg + j * GK;
+                        for (int k = 0; k < K; ++k) s_xchg[lane * K + k] = Lp[k];
+                        __syncthreads();
+                        const int* lineX = s_xchg + j * GK;
 #pragma unroll
-                    for (int k = 0; k < K; ++k) {
-                        const int sm = g * K + k + she;
-                        const int v0 = lineX[imin(imax(sm, 0), D - 1)];
-                        const int vm = lineX[imin(imax(sm - 1, 0), D - 1)];
-                        const int vp = lineX[imin(imax(sm + 1, 0), D - 1)];
-                        l0[k] = (sm >= 0 && sm < D) ? v0 : kMax;
-                        lm[k] = ((sm >= 1 && sm <= D) ? vm : kMax) + P1;
-                        lp[k] = ((sm >= -1 && sm < D - 1) ? vp : kMax) + P1;
+                        for (int k = 0; k < K; ++k) {
+                            const int sm = g * K + k + she;
+                            const int v0 = lineX[imin(imax(sm, 0), D - 1)];
+                            const int vm = lineX[imin(imax(sm - 1, 0), D - 1)];
+                            const int vp = lineX[imin(imax(sm + 1, 0), D - 1)];
+                            l0[k] = (sm >= 0 && sm < D) ? v0 : kMax;
+                            lm[k] = ((sm >= 1 && sm <= D) ? vm : kMax) + P1;
+                            lp[k] = ((sm >= -1 && sm < D - 1) ? vp : kMax) + P1;
+                        }
                     }
                 }
-                prevBase = base;
             }
-            if (!moved) {
+            if (!REFINE || !moved) {
                 const int lmS = from_lower<G>(Lp[K - 1], kMax, g);
                 const int lpS = from_upper<G>(Lp[0], kMax, g);
 #pragma unroll
@@ -317,13 +342,25 @@ __device__ __forceinline__ void runLine(const PathArgs& a, int lineGroup, const 
                     lp[k] = (k < K - 1 ? Lp[k + 1] : lpS) + P1;
                 }
             }
+            const int cm = 0 - minPrev;
 #pragma unroll
             for (int k = 0; k < K; ++k) {
-                const int best = imin(imin(l0[k], lm[k]), imin(lp[k], lj));
-                const int rec = best < kMax ? sat16(c[k] + best - minPrev) : kMax;
-                // Label valid <=> its cost is below SHRT_MAX (the cost kernels store
-                // SHRT_MAX for every geometry-invalid label at pyramid levels).
-                Lc[k] = c[k] < kMax ? rec : kMax;
+                const int m3 = imin(imin(l0[k], lm[k]), lp[k]);
+                if constexpr (EXACT) {
+                    const int best = imin(m3, imin(kMax, minPrev + p2));
+                    int rec = best < kMax ? imin(kMax, c[k] + cm + best) : kMax;
+                    rec = active ? rec : c[k];
+                    // Label valid <=> its cost is below SHRT_MAX (the cost kernels
+                    // store SHRT_MAX for every geometry-invalid label).
+                    Lc[k] = c[k] < kMax ? rec : kMax;
+                } else {
+                    // Fast form: on active lines minPrev + P2' < SHRT_MAX (host-checked)
+                    // so best < SHRT_MAX and L = min(MAX, C + min(m3, minPrev+P2') - minPrev)
+                    // = min3(C + m3 - minPrev, C + P2', MAX); m3 >= minPrev, so an invalid
+                    // label (C = SHRT_MAX) yields SHRT_MAX, and an inactive line
+                    // (all SHRT_MAX, minPrev = SHRT_MAX) yields C.
+                    Lc[k] = imin(imin(c[k] + cm + m3, c[k] + p2), kMax);
+                }
             }
         }
 
@@ -333,7 +370,7 @@ __device__ __forceinline__ void runLine(const PathArgs& a, int lineGroup, const 
         const int minCur = gmin<G>(m);
         bstore<K>(Lc, rOut, vVol, soffOut);
 
-        if constexpr (PYR) {
+        if constexpr (PYR && EXACT) {
             active = minCur < kMax;
 #pragma unroll
             for (int k = 0; k < K; ++k) Lp[k] = active ? Lc[k] : 0;
@@ -346,6 +383,7 @@ __device__ __forceinline__ void runLine(const PathArgs& a, int lineGroup, const 
     };
 
     // Process one prefetched chunk [sc, sc+U).
+    int lastBase = a.minD;
     auto process = [&](const StepIn<K>(&buf)[U], int sc) {
         int p2[U];
 #pragma unroll
@@ -357,11 +395,22 @@ __device__ __forceinline__ void runLine(const PathArgs& a, int lineGroup, const 
             }
         }
         if constexpr (ADAPT) prevIm = buf[U - 1].im;
+        // Steps at which any line's label window moves, known before the chain runs.
+        uint32_t moveMask = 0;
+        if constexpr (REFINE) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int b = (int)(int16_t)(buf[u].lw & 0xFFFF);
+                if (__ballot(b != (u ? (int)(int16_t)(buf[u - 1].lw & 0xFFFF) : lastBase)) != 0)
+                    moveMask |= 1u << u;
+            }
+            lastBase = (int)(int16_t)(buf[U - 1].lw & 0xFFFF);
+        }
         const ChunkAddr<U> co = chunk(sc, volRow, VB, VB);
         const auto rOut = rsrc(reinterpret_cast<char*>(out) + co.origin, co.nr);
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            if (sc + u < s1) step(buf[u], p2[u], rOut, co.soff[u]);
+            if (sc + u < s1) step(buf[u], p2[u], (moveMask >> u) & 1u, rOut, co.soff[u]);
         }
     };
 
@@ -376,7 +425,7 @@ __device__ __forceinline__ void runLine(const PathArgs& a, int lineGroup, const 
     }
 }
 
-template <int G, int K, int MODE, bool ADAPT>
+template <int G, int K, int MODE, bool ADAPT, bool EXACT>
 __global__ __launch_bounds__(64) void kPaths(PathArgs a)
 {
     __shared__ int s_lut[256];
@@ -387,16 +436,16 @@ __global__ __launch_bounds__(64) void kPaths(PathArgs a)
     }
     const int b = blockIdx.x;
     if (b < a.nLRBlocks)
-        runLine<G, K, MODE, ADAPT, kLR>(a, b, s_lut, s_xchg);
+        runLine<G, K, MODE, ADAPT, EXACT, kLR>(a, b, s_lut, s_xchg);
     else if (b < a.nLRBlocks + a.nRLBlocks)
-        runLine<G, K, MODE, ADAPT, kRL>(a, b - a.nLRBlocks, s_lut, s_xchg);
+        runLine<G, K, MODE, ADAPT, EXACT, kRL>(a, b - a.nLRBlocks, s_lut, s_xchg);
     else
-        runLine<G, K, MODE, ADAPT, kTB>(a, b - a.nLRBlocks - a.nRLBlocks, s_lut, s_xchg);
+        runLine<G, K, MODE, ADAPT, EXACT, kTB>(a, b - a.nLRBlocks - a.nRLBlocks, s_lut, s_xchg);
 }
 
 inline int linesPerWave(int Dp) { return Dp <= 16 ? 4 : (Dp <= 32 ? 2 : 1); }
 
-template <int MODE, bool ADAPT>
+template <int MODE, bool ADAPT, bool EXACT>
 hipError_t dispatchPaths(const PathArgs& a, hipStream_t s)
 {
     const int lpw = linesPerWave(a.Dp);
@@ -406,13 +455,13 @@ hipError_t dispatchPaths(const PathArgs& a, hipStream_t s)
     const int nTB = a.numPaths >= 3 ? (a.W + lpw - 1) / lpw : 0;
     const int blocks = b.nLRBlocks + b.nRLBlocks + nTB;
     switch (a.Dp) {
-    case 16: kPaths<16, 1, MODE, ADAPT><<<blocks, 64, 0, s>>>(b); break;
-    case 32: kPaths<32, 1, MODE, ADAPT><<<blocks, 64, 0, s>>>(b); break;
-    case 64: kPaths<64, 1, MODE, ADAPT><<<blocks, 64, 0, s>>>(b); break;
-    case 128: kPaths<64, 2, MODE, ADAPT><<<blocks, 64, 0, s>>>(b); break;
-    case 256: kPaths<64, 4, MODE, ADAPT><<<blocks, 64, 0, s>>>(b); break;
-    case 512: kPaths<64, 8, MODE, ADAPT><<<blocks, 64, 0, s>>>(b); break;
-    case 1024: kPaths<64, 16, MODE, ADAPT><<<blocks, 64, 0, s>>>(b); break;
+    case 16: kPaths<16, 1, MODE, ADAPT, EXACT><<<blocks, 64, 0, s>>>(b); break;
+    case 32: kPaths<32, 1, MODE, ADAPT, EXACT><<<blocks, 64, 0, s>>>(b); break;
+    case 64: kPaths<64, 1, MODE, ADAPT, EXACT><<<blocks, 64, 0, s>>>(b); break;
+    case 128: kPaths<64, 2, MODE, ADAPT, EXACT><<<blocks, 64, 0, s>>>(b); break;
+    case 256: kPaths<64, 4, MODE, ADAPT, EXACT><<<blocks, 64, 0, s>>>(b); break;
+    case 512: kPaths<64, 8, MODE, ADAPT, EXACT><<<blocks, 64, 0, s>>>(b); break;
+    case 1024: kPaths<64, 16, MODE, ADAPT, EXACT><<<blocks, 64, 0, s>>>(b); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -421,7 +470,9 @@ hipError_t dispatchPaths(const PathArgs& a, hipStream_t s)
 template <int MODE>
 hipError_t launchPathsMode(const PathArgs& a, int adaptive, hipStream_t s)
 {
-    return adaptive ? dispatchPaths<MODE, true>(a, s) : dispatchPaths<MODE, false>(a, s);
+    if (MODE != kSingle && a.exact)
+        return adaptive ? dispatchPaths<MODE, true, true>(a, s) : dispatchPaths<MODE, false, true>(a, s);
+    return adaptive ? dispatchPaths<MODE, true, false>(a, s) : dispatchPaths<MODE, false, false>(a, s);
 }
 
 } // namespace path

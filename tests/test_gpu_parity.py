@@ -153,6 +153,37 @@ def test_refine_level_parity(ws, w, h, D, maxD, cost, paths, adaptive, post):
     U.assert_same("conf", cf, want.conf)
 
 
+@pytest.mark.parametrize("refine", [False, True])
+def test_pyramid_exact_form_large_p2(ws, refine):
+    """Cmax + 2*P2' >= SHRT_MAX selects the path variant that keeps the
+    reference's explicit SHRT_MAX tests; it must agree with the oracle too."""
+    w, h = 200, 72
+    left, right = pair(w, h, 60, seed=77)
+    kw = dict(block=5, P1=300, P2=16000, cost=0, post=1, paths=3, adaptive=int(refine))
+    if refine:
+        pl, pr = O.pyrdown(left), O.pyrdown(right)
+        parent = O.compute_level(pl, pr, 32, 31, block=3, P1=300, P2=16000, cost=0, post=1,
+                                 want32=False)
+        kw.update(refine=True, min_conf=0.05)
+        args = (16, 63, parent.disp16, parent.conf)
+    else:
+        args = (64, 63)
+    rc, d16, d32, cf = U.level(ws, left, right, *args, **kw)
+    want = O.compute_level(left, right, *args, **kw)
+    assert rc == 0 and want.rc == 0
+    U.assert_same("disp16", d16, want.disp16)
+    U.assert_same("disp32", d32, want.disp32)
+    U.assert_same("conf", cf, want.conf)
+
+
+def test_rejects_negative_penalties(ws):
+    left, right = pair(64, 32, 10, seed=5)
+    rc, *_ = U.single_scale(ws, left, right, 16, P1=-1, P2=10)
+    assert rc == -1
+    rc, *_ = U.level(ws, left, right, 16, 15, P1=2, P2=-3)
+    assert rc == -1
+
+
 def test_refine_varying_base(ws):
     """Parent disparity alternating 2 / 30 px (reference pyramid_smoke.cpp:115-134)."""
     left, right = pattern(320, 128, 16, False), pattern(320, 128, 16, True)

@@ -55,14 +55,98 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def copy_calibration(nbytes: int) -> int:
+    """A plain elementwise device copy (int32 negate: reads nbytes, writes nbytes)."""
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.int32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        torch.neg(a, out=b)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": nbytes}))
+    return 0
+
+
+# Kernel-name stems of each stage's launches (demangled names in rocprofv3 output).
+STAGE_KERNELS = {"prep": ("kPrep",), "cost": ("kCensusCost", "kCensusLocal", "kBTRow", "kBoxV",
+                                              "kBTLocal"),
+                 "paths": ("kPaths",), "wta": ("kWTA",), "post": ("kPost",),
+                 "pyrdown": ("kPyrDown",)}
+
+
+def _counter_rows(path: str, counter: str):
+    import csv
+    with open(path, newline="") as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"] == counter:
+                yield int(r["Dispatch_Id"]), r["Kernel_Name"], float(r["Counter_Value"])
+
+
+def derive_traffic(bench_json, fetch_csv, write_csv, cal_fetch_csv, cal_write_csv, cal_bytes):
+    """Per-launch HBM bytes of each (stage, level) from FETCH_SIZE / WRITE_SIZE (KiB per
+    dispatch), each scaled by what the same counter reads for a copy of cal_bytes bytes.
+
+    The library's launches are matched, in dispatch order, to the per-frame launch
+    sequence the bench run recorded; the kernel name of each launch is checked against
+    the stage it is matched to."""
+    with open(bench_json) as f:
+        seq = next(json.loads(l) for l in f if l.startswith("{"))["launch_sequence"]
+
+    def scale(cal_csv, counter):
+        return cal_bytes / (max(v for _, n, v in _counter_rows(cal_csv, counter)
+                                if "pyr::" not in n) * 1024.0)
+
+    out = {}
+    for counter, path, key in (("FETCH_SIZE", fetch_csv, "read"), ("WRITE_SIZE", write_csv, "write")):
+        k = scale(cal_write_csv if key == "write" else cal_fetch_csv, counter)
+        rows = sorted(r for r in _counter_rows(path, counter) if "pyr::" in r[1])
+        for i, (_, name, kib) in enumerate(rows):
+            tag = seq[i % len(seq)]
+            stage = tag.split("@")[0]
+            if not any(stem in name for stem in STAGE_KERNELS[stage]):
+                raise SystemExit(f"launch {i} is {name}, expected a {stage} kernel")
+            out.setdefault(tag, {}).setdefault(key, []).append(kib * 1024.0 * k)
+        out["_scale_" + key] = k
+    res = {}
+    for tag, v in out.items():
+        if tag.startswith("_"):
+            res[tag] = round(v, 4)
+            continue
+        rd = sum(v["read"]) / len(v["read"])
+        wr = sum(v["write"]) / len(v["write"])
+        res[tag] = {"read": round(rd), "write": round(wr), "total": round(rd + wr),
+                    "launches": len(v["read"])}
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-reps", type=int, default=6, help="oracle runs in the CPU baseline")
+    ap.add_argument("--cpu-reps", type=int, default=12, help="oracle runs in the CPU baseline")
+    ap.add_argument("--no-kernel-events", action="store_true",
+                    help="time the steps without per-kernel HIP events (no roofline)")
+    ap.add_argument("--breakdown-steps", type=int, default=20,
+                    help="untimed steps with events around every launch (per-kernel table)")
+    ap.add_argument("--copy-calibration", type=int, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (read MIB, write MIB) five "
+                         "times: the reference kernel for scaling FETCH_SIZE / WRITE_SIZE")
+    ap.add_argument("--derive-traffic", nargs=5,
+                    metavar=("BENCH_JSON", "FETCH_CSV", "WRITE_CSV", "CAL_FETCH_CSV",
+                             "CAL_WRITE_CSV"),
+                    help="turn rocprofv3 FETCH_SIZE / WRITE_SIZE counter CSVs of a bench run "
+                         "(and of a --copy-calibration run) into per-launch HBM bytes, "
+                         "printed as JSON")
+    ap.add_argument("--calib-mib", type=int, default=512)
     args = ap.parse_args()
+    if args.derive_traffic:
+        print(json.dumps(derive_traffic(*args.derive_traffic, args.calib_mib << 20), indent=1))
+        return 0
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration << 20)
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "pairs are independent, so a multi-GPU run would shard pairs across GPUs "
@@ -104,8 +188,51 @@ def main() -> int:
 
     L = P.lib()
     L.pyrsgm_prof_start.argtypes = [C.c_int]
+    L.pyrsgm_prof_filter.argtypes = [C.c_int, C.c_int, C.c_int]
     L.pyrsgm_prof_get.argtypes = [C.c_int] + [C.POINTER(C.c_int)] * 4 + [C.POINTER(C.c_float)]
-    L.pyrsgm_prof_start(args.steps * 20)
+
+    def read_records(nsteps):
+        """Recorded (stage, level) -> durations, and one frame's launch order."""
+        agg: dict[tuple, list] = {}
+        seq: list[str] = []
+        st, w_, h_, l_ = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        ms = C.c_float()
+        n = L.pyrsgm_prof_count()
+        for i in range(n):
+            if L.pyrsgm_prof_get(i, C.byref(st), C.byref(w_), C.byref(h_), C.byref(l_),
+                                 C.byref(ms)) == 0:
+                key = (STAGES[st.value], w_.value, h_.value, l_.value)
+                agg.setdefault(key, []).append(ms.value)
+                if i < n // nsteps:
+                    seq.append(f"{key[0]}@{key[1]}x{key[2]}x{key[3]}")
+        return agg, seq
+
+    def kernel_row(key, v):
+        name, kw, kh, kl = key
+        mean = sum(v) / len(v)
+        nbytes = algorithmic_bytes(name, kw, kh, kl)
+        return {"kernel": name, "level": f"{kw}x{kh}x{kl}", "launches": len(v),
+                "mean_us": round(mean * 1000.0, 2), "alg_bytes": nbytes,
+                "alg_GBps": round(nbytes / (mean * 1e-3) / 1e9, 1)}
+
+    # Breakdown pass (untimed): HIP events around every launch.  An event pair between
+    # consecutive launches costs a few microseconds of drain each, so the timed region
+    # below carries events around the dominant kernel only.
+    kernels, seq = [], []
+    if not args.no_kernel_events:
+        nb = max(1, args.breakdown_steps)
+        L.pyrsgm_prof_filter(-1, -1, -1)
+        L.pyrsgm_prof_start(nb * 20)
+        for _ in range(nb):
+            step()
+        torch.cuda.synchronize()
+        L.pyrsgm_prof_stop()
+        agg, seq = read_records(nb)
+        kernels = sorted((kernel_row(k, v) for k, v in agg.items()), key=lambda k: -k["mean_us"])
+        dom = kernels[0]
+        dw, dh = (int(x) for x in dom["level"].split("x")[:2])
+        L.pyrsgm_prof_filter({v: k for k, v in STAGES.items()}[dom["kernel"]], dw, dh)
+        L.pyrsgm_prof_start(args.steps * 4)
 
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
            for _ in range(args.steps)]
@@ -118,34 +245,32 @@ def main() -> int:
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     L.pyrsgm_prof_stop()
+    L.pyrsgm_prof_filter(-1, -1, -1)
     ms_per_step = (t1 - t0) * 1000.0 / args.steps
     frame_ms = sorted(a.elapsed_time(b) for a, b in evs)
     log(f"[bench] timed {args.steps} steps: {ms_per_step:.3f} ms/step wall, "
         f"median event {statistics.median(frame_ms):.3f} ms")
 
-    # Per-kernel durations recorded inside the timed loop.
-    agg: dict[tuple, list] = {}
-    st, w_, h_, l_ = C.c_int(), C.c_int(), C.c_int(), C.c_int()
-    ms = C.c_float()
-    for i in range(L.pyrsgm_prof_count()):
-        if L.pyrsgm_prof_get(i, C.byref(st), C.byref(w_), C.byref(h_), C.byref(l_), C.byref(ms)) == 0:
-            agg.setdefault((STAGES[st.value], w_.value, h_.value, l_.value), []).append(ms.value)
-    kernels = []
-    for (name, kw, kh, kl), v in agg.items():
-        mean = sum(v) / len(v)
-        nbytes = algorithmic_bytes(name, kw, kh, kl)
-        kernels.append({"kernel": name, "level": f"{kw}x{kh}x{kl}", "launches": len(v),
-                        "mean_us": round(mean * 1000.0, 2),
-                        "alg_bytes": nbytes, "alg_GBps": round(nbytes / (mean * 1e-3) / 1e9, 1)})
-    kernels.sort(key=lambda k: -k["mean_us"])
-    dom = kernels[0]
-    achieved = dom["alg_GBps"]
-    traffic = None
-    tpath = os.path.join(here, "profiles", "traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            tj = json.load(f)
-        traffic = tj.get(f"{dom['kernel']}@{dom['level']}")
+    roofline = None
+    if kernels:
+        # Dominant kernel (largest share of the frame), timed inside the timed region.
+        # traffic = its measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE,
+        # each scaled by the same counter on a plain copy) from the committed summary.
+        agg, _ = read_records(args.steps)
+        timed = [kernel_row(k, v) for k, v in agg.items()]
+        if timed:
+            dom = timed[0]
+        traffic = None
+        tpath = os.path.join(here, "profiles", "traffic_1920x1080_d256_census_l2.json")
+        if os.path.exists(tpath):
+            with open(tpath) as f:
+                t = json.load(f).get(f"{dom['kernel']}@{dom['level']}")
+            traffic = t["total"] if t else None
+        roofline = {"bound": "hbm", "achieved": dom["alg_GBps"], "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(dom["alg_GBps"] / HBM_PEAK_GBS, 4),
+                    "traffic": traffic, "kernel": dom["kernel"], "level": dom["level"],
+                    "alg_bytes_per_launch": dom["alg_bytes"], "mean_us": dom["mean_us"],
+                    "launches_timed": dom["launches"]}
 
     # CPU baseline + full-size parity: the oracle on the same pair.
     cpu = None
@@ -193,14 +318,13 @@ def main() -> int:
                    "disp12_max_diff": 1, "median": True, "outputs": "disp16+disp32+conf"},
         "frame_ms_p50": round(statistics.median(frame_ms), 4),
         "frame_ms_p99": round(frame_ms[min(len(frame_ms) - 1, int(0.99 * len(frame_ms)))], 4),
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": dom["kernel"], "level": dom["level"]},
+        "roofline": roofline,
         "cpu_baseline": cpu,
         "parity_vs_oracle": parity,
         "accuracy": {"density": round(float(valid.mean()), 4),
                      "bad1": round(float((err > 1).mean()), 4) if err.size else None},
         "kernels": kernels,
+        "launch_sequence": seq,
     }
     print(json.dumps(out), flush=True)
     m.close()

@@ -24,6 +24,9 @@ extern "C" {
 
 /* Enable recording of up to max_records event pairs (pre-created here). */
 int pyrsgm_prof_start(int max_records);
+/* Record only launches of this stage at this level width/height (-1: any).
+ * Persists across start/stop; (-1, -1, -1) records everything. */
+void pyrsgm_prof_filter(int stage, int width, int height);
 /* Stop recording; recorded pairs stay readable until the next start. */
 void pyrsgm_prof_stop(void);
 int pyrsgm_prof_count(void);

@@ -7,8 +7,9 @@
 // x = gradient channel, y = intensity channel, each (v | v0<<8 | v1<<16).
 // A BT cost is then two interval distances (sgm_kernels.cu:1025-1027, :1042).
 //
-// Volumes are int16 [y][x][Dp], labels innermost; every thread produces 8
-// consecutive labels of one pixel and writes them with one 16-byte store.
+// Volumes are int16 [y][x][Dp], labels innermost, stored as SHRT_MAX - cost;
+// every thread produces 8 consecutive labels of one pixel and writes them with
+// one 16-byte store.
 #include "device_common.h"
 #include "internal.h"
 
@@ -100,6 +101,16 @@ __device__ __forceinline__ void store8(int16_t* dst, const int (&v)[8])
     *reinterpret_cast<uint4*>(dst) = o;
 }
 
+// Matching-cost volumes are stored complemented, E = SHRT_MAX - C (the form
+// the path recurrence consumes): invalid and padding labels hold 0.
+__device__ __forceinline__ void storeCost8(int16_t* dst, const int (&v)[8])
+{
+    int e[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) e[i] = kMax - v[i];
+    store8(dst, e);
+}
+
 // ---- Census cost over the full label range (sgm_kernels.cu:303-327) --------
 // Block: 256 threads = NPX pixels x (Dp/8) label octets of one row segment; the
 // right-view census words the segment can reach are staged in LDS.
@@ -139,7 +150,7 @@ __global__ __launch_bounds__(256) void kCensusCost(const uint2* desc, int16_t* C
         v[i] = d >= D ? kMax : (in ? __popc(cl ^ s_r[x2 - base2]) : 24);
         if (geomMask && (!in || minD + d > hiReal)) v[i] = kMax;
     }
-    store8(C + ((size_t)y * W + x) * Dp + d0, v);
+    storeCost8(C + ((size_t)y * W + x) * Dp + d0, v);
 }
 
 // ---- refinement labels (sgm_kernels.cu:955-1000) ----------------------------
@@ -172,10 +183,18 @@ __device__ __forceinline__ void refineLabel(const RefineIn& r, int x, int y, int
     valid = 1;
 }
 
+// Label word of a refinement pixel: bits 0-15 the window base (int16), bit 16
+// the parent-validity flag, bits 24-31 the left-image intensity, which the
+// adaptive-P2 path step reads from here instead of a separate image load.
+__device__ __forceinline__ int32_t labelWord(int base, int valid, uint8_t im)
+{
+    return (int32_t)(((uint32_t)base & 0xFFFFu) | ((uint32_t)valid << 16) | ((uint32_t)im << 24));
+}
+
 // ---- Census cost at refinement levels (sgm_kernels.cu:1112-1133) ------------
 __global__ __launch_bounds__(256) void kCensusLocal(const uint2* desc, RefineIn r, int32_t* labels,
-                                                    int16_t* C, int W, int H, int D, int Dp,
-                                                    int minD, int maxD)
+                                                    const uint8_t* img, int istep, int16_t* C,
+                                                    int W, int H, int D, int Dp, int minD, int maxD)
 {
     const int tpp = Dp >> 3;
     const int npx = 256 / tpp;
@@ -187,7 +206,7 @@ __global__ __launch_bounds__(256) void kCensusLocal(const uint2* desc, RefineIn 
     int base, valid;
     refineLabel(r, x, y, D, minD, maxD, base, valid);
     const size_t p = (size_t)y * W + x;
-    if (d0 == 0) labels[p] = (base & 0xFFFF) | (valid << 16);
+    if (d0 == 0) labels[p] = labelWord(base, valid, img[(size_t)y * istep + x]);
     const uint32_t cl = desc[p].x;
     const uint2* R = desc + (size_t)W * H + (size_t)y * W;
     const int lo = imax(minD, x - W + 1), hi = imin(maxD, x);
@@ -199,7 +218,7 @@ __global__ __launch_bounds__(256) void kCensusLocal(const uint2* desc, RefineIn 
         const bool ok = valid && d < D && real >= lo && real <= hi;
         v[i] = ok ? __popc(cl ^ R[x - real].x) : kMax;
     }
-    store8(C + p * Dp + d0, v);
+    storeCost8(C + p * Dp + d0, v);
 }
 
 // ---- BT cost + horizontal window sum (sgm_kernels.cu:139-239, :354-379) ------
@@ -279,13 +298,13 @@ __global__ __launch_bounds__(256) void kBoxV(const int16_t* Hs, int16_t* C, int 
         // carry SHRT_MAX (see kCensusCost).
         if (d0 + i >= D || (geomMask && (real > hiReal || x - real < 0 || x - real >= W))) v[i] = kMax;
     }
-    store8(C + p * Dp + d0, v);
+    storeCost8(C + p * Dp + d0, v);
 }
 
 // ---- BT window cost at refinement levels (sgm_kernels.cu:1075-1110) ----------
 __global__ __launch_bounds__(256) void kBTLocal(const uint2* desc, RefineIn r, int32_t* labels,
-                                                int16_t* C, int W, int H, int D, int Dp, int minD,
-                                                int maxD, int half)
+                                                const uint8_t* img, int istep, int16_t* C, int W,
+                                                int H, int D, int Dp, int minD, int maxD, int half)
 {
     const int tpp = Dp >> 3;
     const int npx = 256 / tpp;
@@ -297,7 +316,7 @@ __global__ __launch_bounds__(256) void kBTLocal(const uint2* desc, RefineIn r, i
     int base, valid;
     refineLabel(r, x, y, D, minD, maxD, base, valid);
     const size_t p = (size_t)y * W + x;
-    if (d0 == 0) labels[p] = (base & 0xFFFF) | (valid << 16);
+    if (d0 == 0) labels[p] = labelWord(base, valid, img[(size_t)y * istep + x]);
     const int lo = imax(minD, x - W + 1), hi = imin(maxD, x);
     const uint2* Ld = desc;
     const uint2* Rd = desc + (size_t)W * H;
@@ -326,7 +345,7 @@ __global__ __launch_bounds__(256) void kBTLocal(const uint2* desc, RefineIn r, i
     int v[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) v[i] = ok[i] ? imin(sum[i], kMax - 1) : kMax;
-    store8(C + p * Dp + d0, v);
+    storeCost8(C + p * Dp + d0, v);
 }
 
 // ---- pyrDown (pyrdown.cu:16-52) ------------------------------------------------
@@ -383,11 +402,11 @@ hipError_t launchCost(const LevelCall& c, Workspace& ws, hipStream_t s)
         RefineIn r{c.pdisp, c.pconf, c.pdpitch, c.pcpitch, c.PW, c.PH, c.minConf};
         dim3 grid((c.W + npx - 1) / npx, c.H);
         if (c.cost == 1)
-            kCensusLocal<<<grid, 256, 0, s>>>(ws.desc, r, ws.labels, ws.C, c.W, c.H, c.D, c.Dp,
-                                              c.minD, c.maxD);
+            kCensusLocal<<<grid, 256, 0, s>>>(ws.desc, r, ws.labels, c.left, c.lstep, ws.C, c.W,
+                                              c.H, c.D, c.Dp, c.minD, c.maxD);
         else
-            kBTLocal<<<grid, 256, 0, s>>>(ws.desc, r, ws.labels, ws.C, c.W, c.H, c.D, c.Dp,
-                                          c.minD, c.maxD, half);
+            kBTLocal<<<grid, 256, 0, s>>>(ws.desc, r, ws.labels, c.left, c.lstep, ws.C, c.W, c.H,
+                                          c.D, c.Dp, c.minD, c.maxD, half);
         return hipGetLastError();
     }
     if (c.cost == 1) {

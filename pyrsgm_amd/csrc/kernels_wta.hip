@@ -23,6 +23,7 @@ namespace {
 using path::gmin;
 using path::gumin;
 
+// Path volumes hold H = SHRT_MAX - L; returns the 16 path costs L.
 __device__ __forceinline__ void load16(int (&v)[16], const int16_t* p)
 {
     const uint4 a = reinterpret_cast<const uint4*>(p)[0];
@@ -30,8 +31,8 @@ __device__ __forceinline__ void load16(int (&v)[16], const int16_t* p)
     const uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-        v[2 * i] = (int)(w[i] & 0xFFFFu);
-        v[2 * i + 1] = (int)(w[i] >> 16);
+        v[2 * i] = kMax - (int)(w[i] & 0xFFFFu);
+        v[2 * i + 1] = kMax - (int)(w[i] >> 16);
     }
 }
 

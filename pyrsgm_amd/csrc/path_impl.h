@@ -163,11 +163,86 @@ __device__ __forceinline__ uint32_t gumin(uint32_t v)
     return v;
 }
 
-template <int K>
-struct StepIn {
-    uint32_t c[(K + 1) / 2];
-    int lw;  // refinement label word
-    int im;  // left-image intensity (adaptive P2)
+// Maximum over each group of G consecutive lanes (G in 16..64), returned to
+// every lane of the group.  Inputs are >= 0, so the zero that bound_ctrl feeds
+// to out-of-row lanes never wins.
+template <int G>
+__device__ __forceinline__ int gmax(int v)
+{
+    v = imax(v, dmov<kDppQuadXor1>(v));
+    v = imax(v, dmov<kDppQuadXor2>(v));
+    v = imax(v, dmov<kDppRowHalfMirror>(v));
+    v = imax(v, dmov<kDppRowMirror>(v));
+    if constexpr (G >= 32) {
+        const auto s = __builtin_amdgcn_permlane16_swap((uint32_t)v, (uint32_t)v, false, false);
+        v = imax((int)s[0], (int)s[1]);
+    }
+    if constexpr (G >= 64) {
+        const auto s = __builtin_amdgcn_permlane32_swap((uint32_t)v, (uint32_t)v, false, false);
+        v = imax((int)s[0], (int)s[1]);
+    }
+    return v;
+}
+
+// Lane n of each 16-lane row, broadcast to the row (DPP row_newbcast).
+template <int N>
+__device__ __forceinline__ int rowBcast(int v)
+{
+    return __builtin_amdgcn_mov_dpp(v, 0x150 + N, 0xF, 0xF, true);
+}
+
+template <int N>
+struct IntC { static constexpr int value = N; };
+
+// f(IntC<0>{}), ..., f(IntC<N-1>{}): a compile-time step index for each step.
+template <int N, int I = 0, class F>
+__device__ __forceinline__ void unrollSteps(F& f)
+{
+    if constexpr (I < N) {
+        f(IntC<I>{});
+        unrollSteps<N, I + 1>(f);
+    }
+}
+
+// Value of the next-lower / next-higher lane of the G-lane group; the group's
+// edge lane gets 0 (the complemented "no path" value).
+template <int G>
+__device__ __forceinline__ int lowerOr0(int v, int g)
+{
+    if constexpr (G == 16) {
+        (void)g;
+        return dmov<kDppRowShr1>(v);
+    } else if constexpr (G == 64) {
+        (void)g;
+        return dmov<kDppWaveShr1>(v);
+    } else {
+        const int t = dmov<kDppWaveShr1>(v);
+        return g == 0 ? 0 : t;
+    }
+}
+
+template <int G>
+__device__ __forceinline__ int upperOr0(int v, int g)
+{
+    if constexpr (G == 16) {
+        (void)g;
+        return dmov<kDppRowShl1>(v);
+    } else if constexpr (G == 64) {
+        (void)g;
+        return dmov<kDppWaveShl1>(v);
+    } else {
+        const int t = dmov<kDppWaveShl1>(v);
+        return g == G - 1 ? 0 : t;
+    }
+}
+
+// One chunk of U pixel steps in flight: per-step complemented costs of this
+// lane's labels, and (refinement / adaptive P2) one per-step word per lane,
+// lane slot r (= lane % 16) holding step sc + r of its line.
+template <int K, int U>
+struct StepBuf {
+    uint32_t c[U][(K + 1) / 2];
+    int w;
 };
 
 // Uniform (per wave, per chunk) addressing of one pixel-ordered stream.
@@ -176,11 +251,25 @@ struct ChunkAddr {
     uint64_t origin;  // byte offset of the chunk's lowest-address pixel on line0
     uint32_t soff[U];
     uint32_t nr;      // bytes reachable from origin by the wave's present lines
+    int sLast;        // last step of the chunk inside the line
 };
 
-// MODE: kSingle, kCoarse or kRefine.  EXACT (pyramid only): keep the
-// reference's explicit "no active path" flag and SHRT_MAX tests, needed when
-// Cmax + 2 * max(P2') could reach SHRT_MAX.
+// Volumes are stored complemented: the cost kernels write E = SHRT_MAX - C and
+// the path kernels write H = SHRT_MAX - L.  With H the recurrence reads
+//   H = max(0, H'[d] + f, max(H'[d-1], H'[d+1]) + f - P1, E - P2'),
+//   f = E - max_d H'[d]
+// (H' the previous pixel's values), which is the reference's
+//   L = min(SHRT_MAX, C + min(L'[d], L'[d+-1] + P1, min(SHRT_MAX, minPrev + P2')) - minPrev)
+// term by term: min <-> max, SHRT_MAX <-> 0, minPrev = SHRT_MAX - max H'.
+// A lane outside the line reads 0 (= SHRT_MAX) from DPP's bound control, so
+// the d = 0 / d = D-1 edges need no fill instructions.  The pyramid levels use
+// it whenever Cmax + 2 * max(P2') < SHRT_MAX (host-checked): then an active
+// line's best is below SHRT_MAX and the reference's "best < MAX ? ... : MAX"
+// and "!prevActive -> L = C" cases reduce to the same expression (a line with
+// no active path holds H = 0 everywhere, so H = E).  Otherwise (EXACT) the
+// step keeps the reference's explicit tests in the L domain.
+//
+// MODE: kSingle, kCoarse or kRefine.
 template <int G, int K, int MODE, bool ADAPT, bool EXACT, int DIR>
 __device__ __forceinline__ void runLine(const PathArgs& a, int lineGroup, const int* s_lut,
                                         int* s_xchg)
@@ -188,15 +277,18 @@ __device__ __forceinline__ void runLine(const PathArgs& a, int lineGroup, const 
     constexpr int LPW = 64 / G;
     constexpr int GK = G * K;
     constexpr int U = prefetchSteps<K>();
+    static_assert(U <= 16, "per-step words use one 16-lane row slot per step");
     constexpr bool VERT = DIR == kTB;
     constexpr bool RL = DIR == kRL;
     constexpr bool PYR = MODE != kSingle;
     constexpr bool REFINE = MODE == kRefine;
+    constexpr bool WORDS = REFINE || ADAPT;  // per-step words are loaded
     constexpr int VB = GK * 2;  // bytes of one pixel's label vector
 
     const int lane = threadIdx.x;
     const int j = lane / G;
     const int g = lane - j * G;
+    const int r16 = lane & 15;
     const int W = a.W, H = a.H, D = a.D;
     const int nLines = VERT ? W : H;
     const int line0 = lineGroup * LPW;
@@ -211,17 +303,24 @@ __device__ __forceinline__ void runLine(const PathArgs& a, int lineGroup, const 
     if (s0 >= s1) return;
 
     // Stream layouts: offset(row, col) = row * rowPitch + col * elem.
-    const uint32_t volRow = (uint32_t)W * VB, labRow = (uint32_t)W * 4, imRow = (uint32_t)a.istep;
+    const uint32_t volRow = (uint32_t)W * VB;
+    const uint32_t wRow = REFINE ? (uint32_t)W * 4 : (uint32_t)a.istep;
+    constexpr uint32_t wElem = REFINE ? 4 : 1;
     auto lineStride = [&](uint32_t rowPitch, uint32_t elem) { return VERT ? elem : rowPitch; };
     auto stepStride = [&](uint32_t rowPitch, uint32_t elem) { return VERT ? rowPitch : elem; };
     const uint32_t vVol = lineOk ? (uint32_t)j * lineStride(volRow, VB) + (uint32_t)g * K * 2 : kOffLimit;
-    const uint32_t vLab = lineOk ? (uint32_t)j * lineStride(labRow, 4) : kOffLimit;
-    const uint32_t vImg = lineOk ? (uint32_t)j * lineStride(imRow, 1) : kOffLimit;
+    const uint32_t wLine = (uint32_t)j * lineStride(wRow, wElem);
+    const uint32_t wStep = stepStride(wRow, wElem);
     int16_t* const out = DIR == kLR ? a.Alr : (DIR == kRL ? a.Arl : a.Atb);
 
-    auto chunk = [&](int sc, uint32_t rowPitch, uint32_t elem, uint32_t vecBytes) {
+    // full: the chunk [sc, sc+U) lies inside the line, so every offset is a
+    // compile-time multiple of the step stride (an instruction immediate for
+    // the horizontal paths).  Otherwise steps past the line re-read its last
+    // step; the look-ahead of the last chunk starts at or past s1.
+    auto chunk = [&](int scIn, uint32_t rowPitch, uint32_t elem, uint32_t vecBytes, bool full) {
         ChunkAddr<U> c;
-        const int sLast = imin(sc + U - 1, s1 - 1);
+        const int sc = full ? scIn : imin(scIn, s1 - 1);
+        const int sLast = full ? sc + U - 1 : imin(sc + U - 1, s1 - 1);
         const int so = RL ? sLast : sc;
         const int row = VERT ? so : line0;
         const int col = VERT ? line0 : (RL ? W - 1 - so : so);
@@ -229,199 +328,211 @@ __device__ __forceinline__ void runLine(const PathArgs& a, int lineGroup, const 
         const uint32_t st = stepStride(rowPitch, elem);
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int su = imin(sc + u, s1 - 1);
+            const int su = full ? sc + u : imin(sc + u, s1 - 1);
             c.soff[u] = (uint32_t)(RL ? so - su : su - so) * st;
         }
         c.nr = (uint32_t)(nValid - 1) * lineStride(rowPitch, elem) + (uint32_t)(sLast - sc) * st + vecBytes;
+        c.sLast = sLast;
         return c;
     };
 
-    auto issue = [&](StepIn<K>(&buf)[U], int sc) {
-        const ChunkAddr<U> cv = chunk(sc, volRow, VB, VB);
+    auto issue = [&](StepBuf<K, U>& buf, int scIn, bool full) {
+        if constexpr (WORDS) {
+            // Lane slot r16 loads step sc + r16 of its line; slots past the chunk
+            // or the line, and lanes of absent lines, read 0 (offset out of range).
+            const ChunkAddr<U> cw = chunk(scIn, wRow, wElem, wElem, full);
+            const int sc = full ? scIn : imin(scIn, s1 - 1);
+            const int rel = RL ? cw.sLast - sc - r16 : r16;
+            const bool ok = lineOk && r16 < U && (full || sc + r16 <= cw.sLast);
+            const uint32_t voff = ok ? wLine + (uint32_t)rel * wStep : kOffLimit;
+            const char* base = REFINE ? reinterpret_cast<const char*>(a.labels) : reinterpret_cast<const char*>(a.img);
+            const auto r = rsrc(base + cw.origin, cw.nr);
+            buf.w = REFINE ? (int)__builtin_amdgcn_raw_buffer_load_b32(r, voff, 0, 0)
+                           : (int)__builtin_amdgcn_raw_buffer_load_b8(r, voff, 0, 0);
+        }
+        const ChunkAddr<U> cv = chunk(scIn, volRow, VB, VB, full);
         const auto rC = rsrc(reinterpret_cast<const char*>(a.C) + cv.origin, cv.nr);
 #pragma unroll
-        for (int u = 0; u < U; ++u) bload<K>(buf[u].c, rC, vVol, cv.soff[u]);
-        if constexpr (REFINE) {
-            const ChunkAddr<U> cl = chunk(sc, labRow, 4, 4);
-            const auto r = rsrc(reinterpret_cast<const char*>(a.labels) + cl.origin, cl.nr);
-#pragma unroll
-            for (int u = 0; u < U; ++u)
-                buf[u].lw = (int)__builtin_amdgcn_raw_buffer_load_b32(r, vLab, cl.soff[u], 0);
-        }
-        if constexpr (ADAPT) {
-            const ChunkAddr<U> ci = chunk(sc, imRow, 1, 1);
-            const auto r = rsrc(a.img + ci.origin, ci.nr);
-#pragma unroll
-            for (int u = 0; u < U; ++u)
-                buf[u].im = (int)__builtin_amdgcn_raw_buffer_load_b8(r, vImg, ci.soff[u], 0);
-        }
+        for (int u = 0; u < U; ++u) bload<K>(buf.c[u], rC, vVol, cv.soff[u]);
     };
 
-    // Path state of this lane's line.  Fast form (EXACT == false): a line
-    // without an active path holds SHRT_MAX everywhere, so the next step's
-    // best is SHRT_MAX and min(MAX, C + best - minPrev) = C.  Exact form: a
-    // line without an active path holds zeros and an explicit flag.
-    int Lp[K];
+    // Path state of this lane's labels: complemented H (fast form) or L with
+    // the reference's activity flag (EXACT).
+    int Hs[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) Lp[k] = EXACT ? 0 : kMax;
-    int minPrev = EXACT ? 0 : kMax;
-    int prevBase = a.minD;
-    bool active = false;
-    int prevIm = 0;
+    for (int k = 0; k < K; ++k) Hs[k] = 0;
+    int maxH = 0;                  // fast form: max over the line of Hs
+    int minPrev = 0;               // EXACT
+    bool active = false;           // EXACT
+    int carryBase = a.minD;        // label-window base of the previous step (slot row)
+    int carryIm4 = 0;              // 4 * intensity of the previous step
     const int P1 = a.P1;
 
-    auto step = [&](const StepIn<K>& st, int p2, bool moved, __amdgpu_buffer_rsrc_t rOut,
-                    uint32_t soffOut) {
-        int c[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k) c[k] = unpk<K>(st.c, k);
-        int Lc[K];
-        if constexpr (!PYR) {
-            // L = min(MAX, C + min(m3, min(MAX, minPrev + P2')) - minPrev) with
-            // m3 = min(L[d], L[d-1]+P1, L[d+1]+P1) >= minPrev, split into
-            // min3(C + m3 - minPrev, C + min(MAX - minPrev, P2'), MAX): no select,
-            // and padding labels (C = SHRT_MAX) stay SHRT_MAX.
-            const int lmS = from_lower<G>(Lp[K - 1], kMax, g);
-            const int lpS = from_upper<G>(Lp[0], kMax, g);
-            const int cm = 0 - minPrev;
-            const int q = imin(kMax - minPrev, p2);
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const int Lm = k > 0 ? Lp[k - 1] : lmS;
-                const int Lq = k < K - 1 ? Lp[k + 1] : lpS;
-                const int m3 = imin(imin(Lm + P1, Lq + P1), Lp[k]);
-                Lc[k] = imin(imin(c[k] + cm + m3, c[k] + q), kMax);
-            }
+    // Previous costs of the line re-indexed after its label window moved by
+    // sh labels: l0 = prev[d + sh], y = max(prev[d + sh - 1], prev[d + sh + 1]),
+    // out-of-window labels reading `none`.
+    auto reindex = [&](const int (&prev)[K], int sh, int none, int (&l0)[K], int (&lm)[K],
+                       int (&lp)[K]) {
+        if constexpr (K == 1) {
+            const int sm = g + sh;
+            const int lane0 = j * G;
+            const int i0 = lane0 + imin(imax(sm, 0), D - 1);
+            const int im = lane0 + imin(imax(sm - 1, 0), D - 1);
+            const int ip = lane0 + imin(imax(sm + 1, 0), D - 1);
+            const int v0 = __builtin_amdgcn_ds_bpermute(i0 << 2, prev[0]);
+            const int vm = __builtin_amdgcn_ds_bpermute(im << 2, prev[0]);
+            const int vp = __builtin_amdgcn_ds_bpermute(ip << 2, prev[0]);
+            l0[0] = (sm >= 0 && sm < D) ? v0 : none;
+            lm[0] = (sm >= 1 && sm <= D) ? vm : none;
+            lp[0] = (sm >= -1 && sm < D - 1) ? vp : none;
         } else {
-            int l0[K], lm[K], lp[K];
-            if constexpr (REFINE) {
-                const int base = (int)(int16_t)(st.lw & 0xFFFF);
-                const int sh = base - prevBase;
-                prevBase = base;
-                if (__builtin_expect(moved, 0)) {
-                    // Some line's label window moved: re-index its previous costs.
-                    const int she = (!EXACT || active) ? sh : 0;
-                    if constexpr (K == 1) {
-                        const int sm = g + she;
-                        const int lane0 = j * G;
-                        const int i0 = lane0 + imin(imax(sm, 0), D - 1);
-                        const int im = lane0 + imin(imax(sm - 1, 0), D - 1);
-                        const int ip = lane0 + imin(imax(sm + 1, 0), D - 1);
-                        const int v0 = __builtin_amdgcn_ds_bpermute(i0 << 2, Lp[0]);
-                        const int vm = __builtin_amdgcn_ds_bpermute(im << 2, Lp[0]);
-                        const int vp = __builtin_amdgcn_ds_bpermute(ip << 2, Lp[0]);
-                        l0[0] = (sm >= 0 && sm < D) ? v0 : kMax;
-                        lm[0] = ((sm >= 1 && sm <= D) ? vm : kMax) + P1;
-                        lp[0] = ((sm >= -1 && sm < D - 1) ? vp : kMax) + P1;
-                    } else {
-                        __syncthreads();
+            __syncthreads();
 #pragma unroll
-                        for (int k = 0; k < K; ++k) s_xchg[lane * K + k] = Lp[k];
-                        __syncthreads();
-                        const int* lineX = s_xchg + j * GK;
-#pragma unroll
-                        for (int k = 0; k < K; ++k) {
-                            const int sm = g * K + k + she;
-                            const int v0 = lineX[imin(imax(sm, 0), D - 1)];
-                            const int vm = lineX[imin(imax(sm - 1, 0), D - 1)];
-                            const int vp = lineX[imin(imax(sm + 1, 0), D - 1)];
-                            l0[k] = (sm >= 0 && sm < D) ? v0 : kMax;
-                            lm[k] = ((sm >= 1 && sm <= D) ? vm : kMax) + P1;
-                            lp[k] = ((sm >= -1 && sm < D - 1) ? vp : kMax) + P1;
-                        }
-                    }
-                }
-            }
-            if (!REFINE || !moved) {
-                const int lmS = from_lower<G>(Lp[K - 1], kMax, g);
-                const int lpS = from_upper<G>(Lp[0], kMax, g);
-#pragma unroll
-                for (int k = 0; k < K; ++k) {
-                    l0[k] = Lp[k];
-                    lm[k] = (k > 0 ? Lp[k - 1] : lmS) + P1;
-                    lp[k] = (k < K - 1 ? Lp[k + 1] : lpS) + P1;
-                }
-            }
-            const int cm = 0 - minPrev;
+            for (int k = 0; k < K; ++k) s_xchg[lane * K + k] = prev[k];
+            __syncthreads();
+            const int* lineX = s_xchg + j * GK;
 #pragma unroll
             for (int k = 0; k < K; ++k) {
-                const int m3 = imin(imin(l0[k], lm[k]), lp[k]);
-                if constexpr (EXACT) {
-                    const int best = imin(m3, imin(kMax, minPrev + p2));
-                    int rec = best < kMax ? imin(kMax, c[k] + cm + best) : kMax;
-                    rec = active ? rec : c[k];
-                    // Label valid <=> its cost is below SHRT_MAX (the cost kernels
-                    // store SHRT_MAX for every geometry-invalid label).
-                    Lc[k] = c[k] < kMax ? rec : kMax;
-                } else {
-                    // Fast form: on active lines minPrev + P2' < SHRT_MAX (host-checked)
-                    // so best < SHRT_MAX and L = min(MAX, C + min(m3, minPrev+P2') - minPrev)
-                    // = min3(C + m3 - minPrev, C + P2', MAX); m3 >= minPrev, so an invalid
-                    // label (C = SHRT_MAX) yields SHRT_MAX, and an inactive line
-                    // (all SHRT_MAX, minPrev = SHRT_MAX) yields C.
-                    Lc[k] = imin(imin(c[k] + cm + m3, c[k] + p2), kMax);
-                }
+                const int sm = g * K + k + sh;
+                const int v0 = lineX[imin(imax(sm, 0), D - 1)];
+                const int vm = lineX[imin(imax(sm - 1, 0), D - 1)];
+                const int vp = lineX[imin(imax(sm + 1, 0), D - 1)];
+                l0[k] = (sm >= 0 && sm < D) ? v0 : none;
+                lm[k] = (sm >= 1 && sm <= D) ? vm : none;
+                lp[k] = (sm >= -1 && sm < D - 1) ? vp : none;
             }
         }
+    };
 
-        int m = Lc[0];
+    // Pixel step u of a chunk.  cw: complemented costs; p2: this step's P2'
+    // (line-uniform); shv: per-step window shifts of this lane's line (slot u is
+    // read on moved steps only).
+    auto step = [&](auto uc, const uint32_t (&cw)[(K + 1) / 2], int p2, bool moved, int shv,
+                    __amdgpu_buffer_rsrc_t rOut, uint32_t soffOut) {
+        constexpr int u = decltype(uc)::value;
+        int e[K];
 #pragma unroll
-        for (int k = 1; k < K; ++k) m = imin(m, Lc[k]);
-        const int minCur = gmin<G>(m);
-        bstore<K>(Lc, rOut, vVol, soffOut);
-
-        if constexpr (PYR && EXACT) {
+        for (int k = 0; k < K; ++k) e[k] = unpk<K>(cw, k);
+        int Hc[K];
+        if constexpr (!EXACT) {
+            // Neighbours of an unmoved window; a moved one overrides them (one
+            // uniform branch per step, taken on the rare moved steps).
+            int h0[K], y[K];
+            const int lowN = lowerOr0<G>(Hs[K - 1], g);
+            const int upN = upperOr0<G>(Hs[0], g);
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                h0[k] = Hs[k];
+                y[k] = imax(k > 0 ? Hs[k - 1] : lowN, k < K - 1 ? Hs[k + 1] : upN);
+            }
+            if (REFINE && __builtin_expect(moved, 0)) {
+                int lm[K], lp[K];
+                reindex(Hs, rowBcast<u>(shv), 0, h0, lm, lp);
+#pragma unroll
+                for (int k = 0; k < K; ++k) y[k] = imax(lm[k], lp[k]);
+            }
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const int f = e[k] - maxH;
+                const int t3 = (int)__builtin_elementwise_sub_sat((uint32_t)e[k], (uint32_t)p2);
+                Hc[k] = imax(imax(h0[k] + f, y[k] + f - P1), t3);
+            }
+            int m = Hc[0];
+#pragma unroll
+            for (int k = 1; k < K; ++k) m = imax(m, Hc[k]);
+            maxH = gmax<G>(m);
+#pragma unroll
+            for (int k = 0; k < K; ++k) Hs[k] = Hc[k];
+        } else {
+            // L domain; Hs holds L here (zeros while no path is active).
+            int l0[K], lm[K], lp[K];
+            const int lmS = from_lower<G>(Hs[K - 1], kMax, g);
+            const int lpS = from_upper<G>(Hs[0], kMax, g);
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                l0[k] = Hs[k];
+                lm[k] = k > 0 ? Hs[k - 1] : lmS;
+                lp[k] = k < K - 1 ? Hs[k + 1] : lpS;
+            }
+            if (REFINE && __builtin_expect(moved, 0))
+                reindex(Hs, active ? rowBcast<u>(shv) : 0, kMax, l0, lm, lp);
+            int Lc[K];
+            const int cap = imin(kMax, minPrev + p2);
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const int c = kMax - e[k];
+                const int m3 = imin(imin(l0[k], lm[k] + P1), lp[k] + P1);
+                const int best = imin(m3, cap);
+                int rec = best < kMax ? imin(kMax, c + best - minPrev) : kMax;
+                rec = active ? rec : c;
+                // Label valid <=> its cost is below SHRT_MAX (the cost kernels
+                // store SHRT_MAX for every geometry-invalid label).
+                Lc[k] = c < kMax ? rec : kMax;
+                Hc[k] = kMax - Lc[k];
+            }
+            int m = Lc[0];
+#pragma unroll
+            for (int k = 1; k < K; ++k) m = imin(m, Lc[k]);
+            const int minCur = gmin<G>(m);
             active = minCur < kMax;
 #pragma unroll
-            for (int k = 0; k < K; ++k) Lp[k] = active ? Lc[k] : 0;
+            for (int k = 0; k < K; ++k) Hs[k] = active ? Lc[k] : 0;
             minPrev = active ? minCur : 0;
-        } else {
-#pragma unroll
-            for (int k = 0; k < K; ++k) Lp[k] = Lc[k];
-            minPrev = minCur;
         }
+        bstore<K>(Hc, rOut, vVol, soffOut);
     };
 
-    // Process one prefetched chunk [sc, sc+U).
-    int lastBase = a.minD;
-    auto process = [&](const StepIn<K>(&buf)[U], int sc) {
-        int p2[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            if constexpr (ADAPT) {
-                p2[u] = s_lut[abs(buf[u].im - (u ? buf[u - 1].im : prevIm))];
-            } else {
-                p2[u] = a.P2;
-            }
-        }
-        if constexpr (ADAPT) prevIm = buf[U - 1].im;
-        // Steps at which any line's label window moves, known before the chain runs.
+    // Process one prefetched chunk [sc, sc+U); full: all U steps exist.
+    auto process = [&](const StepBuf<K, U>& buf, int sc, bool full) {
+        int p2v = 0, shv = 0;
         uint32_t moveMask = 0;
         if constexpr (REFINE) {
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int b = (int)(int16_t)(buf[u].lw & 0xFFFF);
-                if (__ballot(b != (u ? (int)(int16_t)(buf[u - 1].lw & 0xFFFF) : lastBase)) != 0)
-                    moveMask |= 1u << u;
-            }
-            lastBase = (int)(int16_t)(buf[U - 1].lw & 0xFFFF);
+            const int base = (int)(int16_t)(buf.w & 0xFFFF);
+            const int prevBase = dpp<kDppRowShr1>(carryBase, base);
+            shv = base - prevBase;
+            const uint64_t bal = __ballot(shv != 0);
+            const uint64_t f2 = bal | (bal >> 32);
+            moveMask = (uint32_t)(f2 | (f2 >> 16)) & 0xFFFFu;
+            carryBase = rowBcast<U - 1>(base);
         }
-        const ChunkAddr<U> co = chunk(sc, volRow, VB, VB);
+        if constexpr (ADAPT) {
+            const int im4 = REFINE ? (int)((uint32_t)buf.w >> 22) : buf.w << 2;
+            const int prevIm4 = dpp<kDppRowShr1>(carryIm4, im4);
+            const uint32_t d4 = __builtin_amdgcn_sad_u16((uint32_t)im4, (uint32_t)prevIm4, 0u);
+            p2v = *reinterpret_cast<const int*>(reinterpret_cast<const char*>(s_lut) + d4);
+            carryIm4 = rowBcast<U - 1>(im4);
+        }
+        const ChunkAddr<U> co = chunk(sc, volRow, VB, VB, full);
         const auto rOut = rsrc(reinterpret_cast<char*>(out) + co.origin, co.nr);
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            if (sc + u < s1) step(buf[u], p2[u], (moveMask >> u) & 1u, rOut, co.soff[u]);
-        }
+        auto run = [&](auto uc) {
+            constexpr int u = decltype(uc)::value;
+            if (full || sc + u < s1) {
+                const int p2 = ADAPT ? rowBcast<u>(p2v) : a.P2;
+                const bool moved = REFINE && ((moveMask >> u) & 1u);
+                step(uc, buf.c[u], p2, moved, shv, rOut, co.soff[u]);
+            }
+        };
+        unrollSteps<U>(run);
     };
 
-    StepIn<K> bufA[U], bufB[U];
-    issue(bufA, s0);
-    for (int sc = s0; sc < s1; sc += 2 * U) {
-        if (sc + U < s1) issue(bufB, sc + U);
-        process(bufA, sc);
-        if (sc + U >= s1) break;
-        if (sc + 2 * U < s1) issue(bufA, sc + 2 * U);
-        process(bufB, sc + U);
+    // Pairs of full chunks run with no conditional loads, stores or steps, so
+    // the loop has one straight path and the compiler's memory-wait counts let
+    // the look-ahead chunk stay in flight (a guarded path would force it to
+    // drain: vmcnt counts loads and stores together).  The remaining steps
+    // follow one chunk at a time.
+    StepBuf<K, U> bufA, bufB;
+    int sc = s0;
+    issue(bufA, sc, sc + U <= s1);
+    for (; sc + 3 * U <= s1; sc += 2 * U) {
+        issue(bufB, sc + U, true);
+        process(bufA, sc, true);
+        issue(bufA, sc + 2 * U, true);
+        process(bufB, sc + U, true);
+    }
+    for (; sc < s1; sc += U) {
+        issue(bufB, sc + U, false);
+        process(bufA, sc, false);
+        bufA = bufB;
     }
 }
 

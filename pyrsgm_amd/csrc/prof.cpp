@@ -15,6 +15,7 @@ struct Record {
 
 struct Prof {
     bool on = false;
+    int fStage = -1, fW = -1, fH = -1;  // record filter; -1 matches anything
     int used = 0;
     std::vector<Record> recs;
     int open = -1;
@@ -32,6 +33,9 @@ void profBegin(int stage, int W, int H, int D, hipStream_t s)
 {
     Prof& p = prof();
     if (!p.on || p.used >= (int)p.recs.size()) return;
+    if ((p.fStage >= 0 && stage != p.fStage) || (p.fW >= 0 && W != p.fW) ||
+        (p.fH >= 0 && H != p.fH))
+        return;
     Record& r = p.recs[p.used];
     r.stage = stage;
     r.W = W;
@@ -73,6 +77,14 @@ int pyrsgm_prof_start(int max_records)
     p.open = -1;
     p.on = true;
     return 0;
+}
+
+void pyrsgm_prof_filter(int stage, int width, int height)
+{
+    Prof& p = prof();
+    p.fStage = stage;
+    p.fW = width;
+    p.fH = height;
 }
 
 void pyrsgm_prof_stop(void) { prof().on = false; }

@@ -20,7 +20,9 @@ enum : int {
     kDppQuadXor1 = 0xB1,       // quad_perm [1,0,3,2]
     kDppQuadXor2 = 0x4E,       // quad_perm [2,3,0,1]
     kDppRowShl1 = 0x101,       // lane i <- lane i+1 inside its 16-lane row
+    kDppRowShl2 = 0x102,       // lane i <- lane i+2 inside its 16-lane row
     kDppRowShr1 = 0x111,       // lane i <- lane i-1 inside its 16-lane row
+    kDppRowShr2 = 0x112,       // lane i <- lane i-2 inside its 16-lane row
     kDppWaveShl1 = 0x130,      // lane i <- lane i+1 across the wave
     kDppWaveShr1 = 0x138,      // lane i <- lane i-1 across the wave
     kDppRowMirror = 0x140,     // lane i <- lane 15-i inside its row

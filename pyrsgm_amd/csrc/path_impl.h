@@ -10,11 +10,13 @@
 // path-cost volume; winner-take-all is a separate, fully parallel kernel.
 //
 // Memory: every stream (costs, label words, left image, path-cost output) is
-// addressed through a buffer resource whose per-step offset is a scalar
-// register, so a step spends no vector instructions on addressing.  The next
-// U steps are loaded while the current U are processed (two register buffers
-// used in turn).  Lanes of lines past the image edge get an offset beyond the
-// buffer's range, so their loads return 0 and their stores are dropped.
+// addressed through a buffer resource whose per-step offset is an instruction
+// immediate or a scalar register, so a step spends no vector instructions on
+// addressing.  Costs are loaded two chunks of U steps ahead (three register
+// buffers in rotation); path costs go to an LDS tile and are written back
+// with 16-byte stores once per chunk.  Lanes of lines past the image edge get
+// an offset beyond the buffer's range, so their loads return 0 and their
+// stores are dropped.
 //
 // Semantics restated from the reference: single scale sgm_kernels.cu:520-578
 // (L->R), :587-638 (T->B), :665-735 (R->L recurrence); pyramid :1191-1261,
@@ -105,6 +107,35 @@ __device__ __forceinline__ void bstore(const int (&v)[K], __amdgpu_buffer_rsrc_t
             o[2] = pk(8 * i + 4);
             o[3] = pk(8 * i + 6);
             __builtin_amdgcn_raw_buffer_store_b128(o, r, voff + 16 * i, soff, 0);
+        }
+    }
+}
+
+// K labels of one lane into the LDS tile (int16 each).
+template <int K>
+__device__ __forceinline__ void tstore(const int (&v)[K], char* p)
+{
+    auto pk = [&](int i) { return (uint32_t)(uint16_t)v[i] | ((uint32_t)v[i + 1] << 16); };
+    if constexpr (K == 1) {
+        *reinterpret_cast<uint16_t*>(p) = (uint16_t)v[0];
+    } else if constexpr (K == 2) {
+        *reinterpret_cast<uint32_t*>(p) = pk(0);
+    } else if constexpr (K == 4) {
+        typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+        u2 o;
+        o[0] = pk(0);
+        o[1] = pk(2);
+        *reinterpret_cast<u2*>(p) = o;
+    } else {
+        typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+        for (int i = 0; i < K / 8; ++i) {
+            u4 o;
+            o[0] = pk(8 * i);
+            o[1] = pk(8 * i + 2);
+            o[2] = pk(8 * i + 4);
+            o[3] = pk(8 * i + 6);
+            reinterpret_cast<u4*>(p)[i] = o;
         }
     }
 }
@@ -269,10 +300,30 @@ struct ChunkAddr {
 // no active path holds H = 0 everywhere, so H = E).  Otherwise (EXACT) the
 // step keeps the reference's explicit tests in the L domain.
 //
+// Path costs of a chunk are staged in an LDS tile and written back with
+// 16-byte stores once per chunk (a 2-byte store per lane and step would cost
+// one memory instruction per step).  Tile layout: horizontal paths
+// [line][step slot][label] with each line's row padded by one pixel's labels
+// (the lines of a wave then write disjoint LDS banks at each step), vertical
+// path [step][line][label].  A horizontal line's row and a vertical step's row are
+// each one contiguous run of the path-cost volume.
+template <int G, int K>
+struct Tile {
+    static constexpr int U = prefetchSteps<K>();
+    static constexpr int LPW = 64 / G;
+    static constexpr int VB = G * K * 2;               // bytes of one pixel's labels
+    static constexpr int ROWH = U * VB + VB;           // horizontal: padded line row, bytes
+    static constexpr int BYTES_H = LPW * ROWH;
+    static constexpr int BYTES_V = U * LPW * VB;
+    static constexpr int BYTES = BYTES_H > BYTES_V ? BYTES_H : BYTES_V;
+    static constexpr int PIECES = LPW * U * VB / 16;   // 16-byte pieces per chunk
+    static_assert(PIECES % 64 == 0, "whole pieces per lane");
+};
+
 // MODE: kSingle, kCoarse or kRefine.
 template <int G, int K, int MODE, bool ADAPT, bool EXACT, int DIR>
 __device__ __forceinline__ void runLine(const PathArgs& a, int lineGroup, const int* s_lut,
-                                        int* s_xchg)
+                                        int* s_xchg, char* s_tile)
 {
     constexpr int LPW = 64 / G;
     constexpr int GK = G * K;
@@ -284,6 +335,7 @@ __device__ __forceinline__ void runLine(const PathArgs& a, int lineGroup, const 
     constexpr bool REFINE = MODE == kRefine;
     constexpr bool WORDS = REFINE || ADAPT;  // per-step words are loaded
     constexpr int VB = GK * 2;  // bytes of one pixel's label vector
+    using TL = Tile<G, K>;
 
     const int lane = threadIdx.x;
     const int j = lane / G;
@@ -406,9 +458,9 @@ __device__ __forceinline__ void runLine(const PathArgs& a, int lineGroup, const 
 
     // Pixel step u of a chunk.  cw: complemented costs; p2: this step's P2'
     // (line-uniform); shv: per-step window shifts of this lane's line (slot u is
-    // read on moved steps only).
-    auto step = [&](auto uc, const uint32_t (&cw)[(K + 1) / 2], int p2, bool moved, int shv,
-                    __amdgpu_buffer_rsrc_t rOut, uint32_t soffOut) {
+    // read on moved steps only); bigMask bit u: some line shifts by more than 1.
+    auto step = [&](auto uc, const uint32_t (&cw)[(K + 1) / 2], int p2, bool moved,
+                    uint32_t bigMask, int shv, int tileByte) {
         constexpr int u = decltype(uc)::value;
         int e[K];
 #pragma unroll
@@ -426,10 +478,23 @@ __device__ __forceinline__ void runLine(const PathArgs& a, int lineGroup, const 
                 y[k] = imax(k > 0 ? Hs[k - 1] : lowN, k < K - 1 ? Hs[k + 1] : upN);
             }
             if (REFINE && __builtin_expect(moved, 0)) {
-                int lm[K], lp[K];
-                reindex(Hs, rowBcast<u>(shv), 0, h0, lm, lp);
+                if (G == 16 && K == 1 && !((bigMask >> u) & 1u)) {
+                    // Every line of the wave moved by -1, 0 or +1 labels: the
+                    // re-indexed values are row shifts of the previous ones (zero
+                    // outside the 16 labels), selected per line.
+                    const int sh = rowBcast<u>(shv);
+                    const int tm2 = dmov<kDppRowShr2>(Hs[0]), tp2 = dmov<kDppRowShl2>(Hs[0]);
+                    const bool neg = sh < 0, pos = sh > 0;
+                    h0[0] = neg ? lowN : (pos ? upN : Hs[0]);
+                    const int lm = neg ? tm2 : (pos ? Hs[0] : lowN);
+                    const int lp = neg ? Hs[0] : (pos ? tp2 : upN);
+                    y[0] = imax(lm, lp);
+                } else {
+                    int lm[K], lp[K];
+                    reindex(Hs, rowBcast<u>(shv), 0, h0, lm, lp);
 #pragma unroll
-                for (int k = 0; k < K; ++k) y[k] = imax(lm[k], lp[k]);
+                    for (int k = 0; k < K; ++k) y[k] = imax(lm[k], lp[k]);
+                }
             }
 #pragma unroll
             for (int k = 0; k < K; ++k) {
@@ -479,20 +544,49 @@ __device__ __forceinline__ void runLine(const PathArgs& a, int lineGroup, const 
             for (int k = 0; k < K; ++k) Hs[k] = active ? Lc[k] : 0;
             minPrev = active ? minCur : 0;
         }
-        bstore<K>(Hc, rOut, vVol, soffOut);
+        tstore<K>(Hc, s_tile + tileByte);
+    };
+
+    // Write the chunk's tile to the path-cost volume: lane l stores 16-byte
+    // pieces l, l+64, ...  A piece belongs to one tile row (a line of a
+    // horizontal path, a step of the vertical one), which is one contiguous run
+    // of the volume; pieces of absent lines or steps past the line get an
+    // out-of-range offset and are dropped.
+    auto flush = [&](int sc, int nSteps, bool full) {
+        const ChunkAddr<U> co = chunk(sc, volRow, VB, VB, full);
+        const auto rOut = rsrc(reinterpret_cast<char*>(out) + co.origin, co.nr);
+        constexpr int PPR = VERT ? LPW * VB / 16 : U * VB / 16;  // pieces per tile row
+        const int nRows = VERT ? nSteps : nValid;
+        const int nCols = VERT ? nValid : nSteps;                // pixels present in a row
+#pragma unroll
+        for (int i = 0; i < TL::PIECES / 64; ++i) {
+            const int pc = lane + 64 * i;
+            const int row = pc / PPR;
+            const int ob = (pc - row * PPR) * 16;                 // byte offset inside the row
+            const int tbyte = row * (VERT ? LPW * VB : TL::ROWH) + ob;
+            typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+            const u4 v = *reinterpret_cast<const u4*>(s_tile + tbyte);
+            const bool ok = row < nRows && ob / VB < nCols;
+            const uint32_t go = ok ? (uint32_t)row * volRow + (uint32_t)ob : kOffLimit;
+            __builtin_amdgcn_raw_buffer_store_b128(v, rOut, go, 0, 0);
+        }
     };
 
     // Process one prefetched chunk [sc, sc+U); full: all U steps exist.
     auto process = [&](const StepBuf<K, U>& buf, int sc, bool full) {
         int p2v = 0, shv = 0;
-        uint32_t moveMask = 0;
+        uint32_t moveMask = 0, bigMask = 0;
         if constexpr (REFINE) {
             const int base = (int)(int16_t)(buf.w & 0xFFFF);
             const int prevBase = dpp<kDppRowShr1>(carryBase, base);
             shv = base - prevBase;
-            const uint64_t bal = __ballot(shv != 0);
-            const uint64_t f2 = bal | (bal >> 32);
-            moveMask = (uint32_t)(f2 | (f2 >> 16)) & 0xFFFFu;
+            // Bit u: some line of the wave moves at step u (any shift / |shift| > 1).
+            auto fold = [](uint64_t b) {
+                const uint64_t f2 = b | (b >> 32);
+                return (uint32_t)(f2 | (f2 >> 16)) & 0xFFFFu;
+            };
+            moveMask = fold(__ballot(shv != 0));
+            bigMask = fold(__ballot(shv > 1 || shv < -1));
             carryBase = rowBcast<U - 1>(base);
         }
         if constexpr (ADAPT) {
@@ -502,37 +596,53 @@ __device__ __forceinline__ void runLine(const PathArgs& a, int lineGroup, const 
             p2v = *reinterpret_cast<const int*>(reinterpret_cast<const char*>(s_lut) + d4);
             carryIm4 = rowBcast<U - 1>(im4);
         }
-        const ChunkAddr<U> co = chunk(sc, volRow, VB, VB, full);
-        const auto rOut = rsrc(reinterpret_cast<char*>(out) + co.origin, co.nr);
+        const int nSteps = full ? U : imin(U, s1 - sc);
         auto run = [&](auto uc) {
             constexpr int u = decltype(uc)::value;
-            if (full || sc + u < s1) {
+            if (full || u < nSteps) {
                 const int p2 = ADAPT ? rowBcast<u>(p2v) : a.P2;
                 const bool moved = REFINE && ((moveMask >> u) & 1u);
-                step(uc, buf.c[u], p2, moved, shv, rOut, co.soff[u]);
+                // Tile slot of step u: horizontal rows run in address order (R->L
+                // reversed), the vertical tile is step-major.
+                int tb;
+                if constexpr (VERT)
+                    tb = (u * LPW + j) * VB + g * K * 2;
+                else
+                    tb = j * TL::ROWH + (RL ? (full ? U - 1 - u : nSteps - 1 - u) : u) * VB + g * K * 2;
+                step(uc, buf.c[u], p2, moved, bigMask, shv, tb);
             }
         };
         unrollSteps<U>(run);
     };
 
-    // Pairs of full chunks run with no conditional loads, stores or steps, so
-    // the loop has one straight path and the compiler's memory-wait counts let
-    // the look-ahead chunk stay in flight (a guarded path would force it to
-    // drain: vmcnt counts loads and stores together).  The remaining steps
-    // follow one chunk at a time.
-    StepBuf<K, U> bufA, bufB;
+    // Three register buffers in rotation: chunk k's loads are issued before
+    // chunk k-2 is processed, so two chunks of steps hide their latency, and
+    // each chunk's tile is written back right after its steps.  Triples of full
+    // chunks run with no conditional loads, stores or steps (a guarded path
+    // makes the compiler's wait counts drain the look-ahead: vmcnt retires
+    // loads and stores together, in issue order); the remaining steps follow
+    // one chunk at a time.
+    StepBuf<K, U> bufA, bufB, bufC;
     int sc = s0;
     issue(bufA, sc, sc + U <= s1);
-    for (; sc + 3 * U <= s1; sc += 2 * U) {
-        issue(bufB, sc + U, true);
+    issue(bufB, sc + U, sc + 2 * U <= s1);
+    for (; sc + 5 * U <= s1; sc += 3 * U) {
+        issue(bufC, sc + 2 * U, true);
         process(bufA, sc, true);
-        issue(bufA, sc + 2 * U, true);
+        flush(sc, U, true);
+        issue(bufA, sc + 3 * U, true);
         process(bufB, sc + U, true);
+        flush(sc + U, U, true);
+        issue(bufB, sc + 4 * U, true);
+        process(bufC, sc + 2 * U, true);
+        flush(sc + 2 * U, U, true);
     }
     for (; sc < s1; sc += U) {
-        issue(bufB, sc + U, false);
+        issue(bufC, sc + 2 * U, false);
         process(bufA, sc, false);
+        flush(sc, imin(U, s1 - sc), false);
         bufA = bufB;
+        bufB = bufC;
     }
 }
 
@@ -541,17 +651,18 @@ __global__ __launch_bounds__(64) void kPaths(PathArgs a)
 {
     __shared__ int s_lut[256];
     __shared__ int s_xchg[MODE == kRefine ? 64 * K : 1];
+    __shared__ __attribute__((aligned(16))) char s_tile[Tile<G, K>::BYTES];
     if constexpr (ADAPT) {
         for (int i = threadIdx.x; i < 256; i += 64) s_lut[i] = a.lut[i];
         __syncthreads();
     }
     const int b = blockIdx.x;
     if (b < a.nLRBlocks)
-        runLine<G, K, MODE, ADAPT, EXACT, kLR>(a, b, s_lut, s_xchg);
+        runLine<G, K, MODE, ADAPT, EXACT, kLR>(a, b, s_lut, s_xchg, s_tile);
     else if (b < a.nLRBlocks + a.nRLBlocks)
-        runLine<G, K, MODE, ADAPT, EXACT, kRL>(a, b - a.nLRBlocks, s_lut, s_xchg);
+        runLine<G, K, MODE, ADAPT, EXACT, kRL>(a, b - a.nLRBlocks, s_lut, s_xchg, s_tile);
     else
-        runLine<G, K, MODE, ADAPT, EXACT, kTB>(a, b - a.nLRBlocks - a.nRLBlocks, s_lut, s_xchg);
+        runLine<G, K, MODE, ADAPT, EXACT, kTB>(a, b - a.nLRBlocks - a.nRLBlocks, s_lut, s_xchg, s_tile);
 }
 
 inline int linesPerWave(int Dp) { return Dp <= 16 ? 4 : (Dp <= 32 ? 2 : 1); }

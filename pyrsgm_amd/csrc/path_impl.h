@@ -12,11 +12,11 @@
 // Memory: every stream (costs, label words, left image, path-cost output) is
 // addressed through a buffer resource whose per-step offset is an instruction
 // immediate or a scalar register, so a step spends no vector instructions on
-// addressing.  Costs are loaded two chunks of U steps ahead (three register
-// buffers in rotation); path costs go to an LDS tile and are written back
-// with 16-byte stores once per chunk.  Lanes of lines past the image edge get
-// an offset beyond the buffer's range, so their loads return 0 and their
-// stores are dropped.
+// addressing.  A second wave per workgroup moves all data: it loads costs two
+// chunks of U steps ahead into LDS and writes the finished path costs back
+// with 16-byte stores, so the recurrence's wave issues no global memory
+// instructions.  Lanes of lines past the image edge get an offset beyond the
+// buffer's range, so their loads return 0 and their stores are dropped.
 //
 // Semantics restated from the reference: single scale sgm_kernels.cu:520-578
 // (L->R), :587-638 (T->B), :665-735 (R->L recurrence); pyramid :1191-1261,
@@ -267,15 +267,6 @@ __device__ __forceinline__ int upperOr0(int v, int g)
     }
 }
 
-// One chunk of U pixel steps in flight: per-step complemented costs of this
-// lane's labels, and (refinement / adaptive P2) one per-step word per lane,
-// lane slot r (= lane % 16) holding step sc + r of its line.
-template <int K, int U>
-struct StepBuf {
-    uint32_t c[U][(K + 1) / 2];
-    int w;
-};
-
 // Uniform (per wave, per chunk) addressing of one pixel-ordered stream.
 template <int U>
 struct ChunkAddr {
@@ -300,30 +291,65 @@ struct ChunkAddr {
 // no active path holds H = 0 everywhere, so H = E).  Otherwise (EXACT) the
 // step keeps the reference's explicit tests in the L domain.
 //
-// Path costs of a chunk are staged in an LDS tile and written back with
-// 16-byte stores once per chunk (a 2-byte store per lane and step would cost
-// one memory instruction per step).  Tile layout: horizontal paths
-// [line][step slot][label] with each line's row padded by one pixel's labels
-// (the lines of a wave then write disjoint LDS banks at each step), vertical
-// path [step][line][label].  A horizontal line's row and a vertical step's row are
-// each one contiguous run of the path-cost volume.
+// A workgroup is two waves.  The compute wave runs the recurrence of 64/G
+// lines and touches only registers and LDS; the memory wave streams the
+// chunk's costs and per-step words into LDS two chunks ahead and writes the
+// finished path costs back.  Each chunk of U steps ends with one workgroup
+// barrier.  LDS plan:
+//  * costs: 3 slots (chunks k, k+1 in use, k+2 being filled); in a slot each
+//    compute lane owns 16 dwords (its K labels for the U steps, a dword per
+//    label pair, or per label when K = 1), padded to 80 bytes so that the
+//    lane's four 16-byte reads hit disjoint banks;
+//  * per-step words: 3 slots of 64 dwords, [line][step slot];
+//  * path costs: 2 tiles (chunk k being written, chunk k-1 being stored).
+//    Horizontal paths [line][step slot][label], each line's row padded by one
+//    pixel's labels (the lines of a wave then write disjoint banks at each
+//    step); vertical path [step][line][label].  A horizontal line's row and a
+//    vertical step's row are each one contiguous run of the path-cost volume,
+//    written with 16-byte stores.
 template <int G, int K>
-struct Tile {
+struct Lds {
     static constexpr int U = prefetchSteps<K>();
     static constexpr int LPW = 64 / G;
     static constexpr int VB = G * K * 2;               // bytes of one pixel's labels
-    static constexpr int ROWH = U * VB + VB;           // horizontal: padded line row, bytes
-    static constexpr int BYTES_H = LPW * ROWH;
-    static constexpr int BYTES_V = U * LPW * VB;
-    static constexpr int BYTES = BYTES_H > BYTES_V ? BYTES_H : BYTES_V;
+    static constexpr int W32 = (K + 1) / 2;            // cost dwords per lane and step
+    static_assert(U * W32 == 16, "a compute lane's chunk of costs is 16 dwords");
+    static constexpr int LANE = 80;                    // bytes per compute lane in a cost slot
+    static constexpr int CSLOT = 64 * LANE;
+    static constexpr int WSLOT = 64 * 4;
+    static constexpr int ROWH = U * VB + VB;           // horizontal tile row, bytes
+    static constexpr int TSLOT = (LPW * ROWH > U * LPW * VB) ? LPW * ROWH : U * LPW * VB;
     static constexpr int PIECES = LPW * U * VB / 16;   // 16-byte pieces per chunk
-    static_assert(PIECES % 64 == 0, "whole pieces per lane");
+    static_assert(PIECES % 64 == 0, "whole pieces per memory lane");
+    static constexpr int NQ = PIECES / 64;
+    static constexpr int OFF_W = 3 * CSLOT;
+    static constexpr int OFF_T = OFF_W + 3 * WSLOT;
+    static constexpr int BYTES = OFF_T + 2 * TSLOT;
 };
+
+// Workgroup barrier ordering LDS only: the memory wave's global loads stay in
+// flight across it.
+__device__ __forceinline__ void wgBarrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+typedef uint32_t u4v __attribute__((ext_vector_type(4)));
+
+// Orders this wave's own LDS accesses (compiler-level; the hardware executes a
+// wave's LDS operations in order).
+__device__ __forceinline__ void waveLdsFence()
+{
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront", "local");
+    asm volatile("" ::: "memory");
+}
 
 // MODE: kSingle, kCoarse or kRefine.
 template <int G, int K, int MODE, bool ADAPT, bool EXACT, int DIR>
 __device__ __forceinline__ void runLine(const PathArgs& a, int lineGroup, const int* s_lut,
-                                        int* s_xchg, char* s_tile)
+                                        int* s_xchg, char* s_lds)
 {
     constexpr int LPW = 64 / G;
     constexpr int GK = G * K;
@@ -335,9 +361,11 @@ __device__ __forceinline__ void runLine(const PathArgs& a, int lineGroup, const 
     constexpr bool REFINE = MODE == kRefine;
     constexpr bool WORDS = REFINE || ADAPT;  // per-step words are loaded
     constexpr int VB = GK * 2;  // bytes of one pixel's label vector
-    using TL = Tile<G, K>;
+    using TL = Lds<G, K>;
+    constexpr int W32 = TL::W32;
 
-    const int lane = threadIdx.x;
+    const bool memWave = threadIdx.x >= 64;
+    const int lane = threadIdx.x & 63;
     const int j = lane / G;
     const int g = lane - j * G;
     const int r16 = lane & 15;
@@ -388,25 +416,144 @@ __device__ __forceinline__ void runLine(const PathArgs& a, int lineGroup, const 
         return c;
     };
 
-    auto issue = [&](StepBuf<K, U>& buf, int scIn, bool full) {
-        if constexpr (WORDS) {
-            // Lane slot r16 loads step sc + r16 of its line; slots past the chunk
-            // or the line, and lanes of absent lines, read 0 (offset out of range).
-            const ChunkAddr<U> cw = chunk(scIn, wRow, wElem, wElem, full);
-            const int sc = full ? scIn : imin(scIn, s1 - 1);
-            const int rel = RL ? cw.sLast - sc - r16 : r16;
-            const bool ok = lineOk && r16 < U && (full || sc + r16 <= cw.sLast);
-            const uint32_t voff = ok ? wLine + (uint32_t)rel * wStep : kOffLimit;
-            const char* base = REFINE ? reinterpret_cast<const char*>(a.labels) : reinterpret_cast<const char*>(a.img);
-            const auto r = rsrc(base + cw.origin, cw.nr);
-            buf.w = REFINE ? (int)__builtin_amdgcn_raw_buffer_load_b32(r, voff, 0, 0)
-                           : (int)__builtin_amdgcn_raw_buffer_load_b8(r, voff, 0, 0);
-        }
-        const ChunkAddr<U> cv = chunk(scIn, volRow, VB, VB, full);
+    const int nChunks = (s1 - s0 + U - 1) / U;
+    const int nFull = (s1 - s0) / U;
+    auto chunkStart = [&](int k) { return s0 + k * U; };
+    auto chunkFull = [&](int k) { return chunkStart(k) + U <= s1; };
+
+    // ---- memory wave ------------------------------------------------------
+    // Piece pc (16 bytes) of a chunk: tile row pc / PPR (a horizontal line or a
+    // vertical step), byte ob inside that row.  The same mapping addresses the
+    // cost loads and the path-cost stores.
+    constexpr int PPR = VERT ? LPW * VB / 16 : U * VB / 16;
+    struct MBuf {
+        u4v q[TL::NQ];
+        int w;
+    };
+    auto mload = [&](MBuf& m, int k) {
+        const int sc = chunkStart(k);
+        const bool full = chunkFull(k);
+        const int nSteps = imin(U, s1 - sc);
+        const int nRows = VERT ? nSteps : nValid;
+        const int nCols = VERT ? nValid : nSteps;
+        const ChunkAddr<U> cv = chunk(sc, volRow, VB, VB, full);
         const auto rC = rsrc(reinterpret_cast<const char*>(a.C) + cv.origin, cv.nr);
 #pragma unroll
-        for (int u = 0; u < U; ++u) bload<K>(buf.c[u], rC, vVol, cv.soff[u]);
+        for (int i = 0; i < TL::NQ; ++i) {
+            const int pc = lane + 64 * i;
+            const int row = pc / PPR;
+            const int ob = (pc - row * PPR) * 16;
+            const bool ok = row < nRows && ob / VB < nCols;
+            const uint32_t go = ok ? (uint32_t)row * volRow + (uint32_t)ob : kOffLimit;
+            m.q[i] = __builtin_amdgcn_raw_buffer_load_b128(rC, go, 0, 0);
+        }
+        if constexpr (WORDS) {
+            // Lane (line jm, slot r) loads the word of step sc + r; slots past the
+            // chunk or the line, and absent lines, read 0 (offset out of range).
+            const ChunkAddr<U> cw = chunk(sc, wRow, wElem, wElem, full);
+            const int jm = lane >> 4, r = lane & 15;
+            const int rel = RL ? cw.sLast - sc - r : r;
+            const bool ok = jm < nValid && r < nSteps;
+            const uint32_t voff = ok ? (uint32_t)jm * lineStride(wRow, wElem) + (uint32_t)rel * wStep : kOffLimit;
+            const char* base = REFINE ? reinterpret_cast<const char*>(a.labels) : reinterpret_cast<const char*>(a.img);
+            const auto rw = rsrc(base + cw.origin, cw.nr);
+            m.w = REFINE ? (int)__builtin_amdgcn_raw_buffer_load_b32(rw, voff, 0, 0)
+                         : (int)__builtin_amdgcn_raw_buffer_load_b8(rw, voff, 0, 0);
+        }
     };
+    // Scatter a loaded chunk into cost slot k % 3: label d of step u goes to
+    // compute lane (line * G + d / K), dword u * W32 + (d % K) / 2.
+    auto mstore = [&](const MBuf& m, int k) {
+        const int sc = chunkStart(k);
+        const int nSteps = imin(U, s1 - sc);
+        const int nRows = VERT ? nSteps : nValid;
+        const int nCols = VERT ? nValid : nSteps;
+        char* const slot = s_lds + (k % 3) * TL::CSLOT;
+#pragma unroll
+        for (int i = 0; i < TL::NQ; ++i) {
+            const int pc = lane + 64 * i;
+            const int row = pc / PPR;
+            const int ob = (pc - row * PPR) * 16;
+            const int col = ob / VB;
+            if (row < nRows && col < nCols) {
+                const int jj = VERT ? col : row;
+                const int u = VERT ? row : (RL ? nSteps - 1 - col : col);
+                const int d0 = (ob - col * VB) / 2;
+                char* const p = slot + (jj * G + d0 / K) * TL::LANE;
+                const u4v q = m.q[i];
+                if constexpr (K == 1) {
+#pragma unroll
+                    for (int t = 0; t < 8; ++t)
+                        *reinterpret_cast<uint32_t*>(p + t * TL::LANE + u * 4) =
+                            (t & 1) ? (q[t >> 1] >> 16) : (q[t >> 1] & 0xFFFFu);
+                } else if constexpr (K == 2) {
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) *reinterpret_cast<uint32_t*>(p + t * TL::LANE + u * 4) = q[t];
+                } else if constexpr (K == 4) {
+                    typedef uint32_t u2v __attribute__((ext_vector_type(2)));
+                    *reinterpret_cast<u2v*>(p + u * 8) = u2v{q[0], q[1]};
+                    *reinterpret_cast<u2v*>(p + TL::LANE + u * 8) = u2v{q[2], q[3]};
+                } else {
+                    *reinterpret_cast<u4v*>(p + u * W32 * 4 + ((d0 % K) / 2) * 4) = q;
+                }
+            }
+        }
+        if constexpr (WORDS) *reinterpret_cast<int*>(s_lds + TL::OFF_W + (k % 3) * TL::WSLOT + lane * 4) = m.w;
+    };
+    // Write path-cost tile k % 2 to the volume; pieces of absent lines or of
+    // steps past the line get an out-of-range offset and are dropped.
+    auto flushTile = [&](int k) {
+        const int sc = chunkStart(k);
+        const bool full = chunkFull(k);
+        const int nSteps = imin(U, s1 - sc);
+        const int nRows = VERT ? nSteps : nValid;
+        const int nCols = VERT ? nValid : nSteps;
+        const char* const tile = s_lds + TL::OFF_T + (k & 1) * TL::TSLOT;
+        const ChunkAddr<U> co = chunk(sc, volRow, VB, VB, full);
+        const auto rOut = rsrc(reinterpret_cast<char*>(out) + co.origin, co.nr);
+#pragma unroll
+        for (int i = 0; i < TL::NQ; ++i) {
+            const int pc = lane + 64 * i;
+            const int row = pc / PPR;
+            const int ob = (pc - row * PPR) * 16;
+            const int tbyte = row * (VERT ? LPW * VB : TL::ROWH) + ob;
+            const u4v v = *reinterpret_cast<const u4v*>(tile + tbyte);
+            const bool ok = row < nRows && ob / VB < nCols;
+            const uint32_t go = ok ? (uint32_t)row * volRow + (uint32_t)ob : kOffLimit;
+            __builtin_amdgcn_raw_buffer_store_b128(v, rOut, go, 0, 0);
+        }
+    };
+
+    if (memWave) {
+        // Chunk c's loads are issued two iterations before it is scattered
+        // into LDS (register sets R0 / R1 by chunk parity); iteration k
+        // (between barriers k and k+1) fills slot k+2 and stores tile k-1.
+        MBuf R0, R1;
+        mload(R0, 0);
+        if (nChunks > 1) mload(R1, 1);
+        mstore(R0, 0);
+        if (nChunks > 1) mstore(R1, 1);
+        if (nChunks > 2) mload(R0, 2);
+        if (nChunks > 3) mload(R1, 3);
+        wgBarrier();
+        int k = 0;
+        for (; k + 1 < nChunks; k += 2) {
+            if (k + 2 < nChunks) mstore(R0, k + 2);
+            if (k + 4 < nChunks) mload(R0, k + 4);
+            if (k >= 1) flushTile(k - 1);
+            wgBarrier();
+            if (k + 3 < nChunks) mstore(R1, k + 3);
+            if (k + 5 < nChunks) mload(R1, k + 5);
+            flushTile(k);
+            wgBarrier();
+        }
+        if (k < nChunks) {
+            if (k >= 1) flushTile(k - 1);
+            wgBarrier();
+        }
+        flushTile(nChunks - 1);
+        return;
+    }
 
     // Path state of this lane's labels: complemented H (fast form) or L with
     // the reference's activity flag (EXACT).
@@ -438,10 +585,13 @@ __device__ __forceinline__ void runLine(const PathArgs& a, int lineGroup, const 
             lm[0] = (sm >= 1 && sm <= D) ? vm : none;
             lp[0] = (sm >= -1 && sm < D - 1) ? vp : none;
         } else {
-            __syncthreads();
+            // Only this wave uses s_xchg, and one wave's LDS operations complete
+            // in issue order: the fences just keep the compiler from moving the
+            // exchange's reads and writes across each other.
+            waveLdsFence();
 #pragma unroll
             for (int k = 0; k < K; ++k) s_xchg[lane * K + k] = prev[k];
-            __syncthreads();
+            waveLdsFence();
             const int* lineX = s_xchg + j * GK;
 #pragma unroll
             for (int k = 0; k < K; ++k) {
@@ -460,7 +610,7 @@ __device__ __forceinline__ void runLine(const PathArgs& a, int lineGroup, const 
     // (line-uniform); shv: per-step window shifts of this lane's line (slot u is
     // read on moved steps only); bigMask bit u: some line shifts by more than 1.
     auto step = [&](auto uc, const uint32_t (&cw)[(K + 1) / 2], int p2, bool moved,
-                    uint32_t bigMask, int shv, int tileByte) {
+                    uint32_t bigMask, int shv, char* tp) {
         constexpr int u = decltype(uc)::value;
         int e[K];
 #pragma unroll
@@ -544,40 +694,31 @@ __device__ __forceinline__ void runLine(const PathArgs& a, int lineGroup, const 
             for (int k = 0; k < K; ++k) Hs[k] = active ? Lc[k] : 0;
             minPrev = active ? minCur : 0;
         }
-        tstore<K>(Hc, s_tile + tileByte);
+        tstore<K>(Hc, tp);
     };
 
-    // Write the chunk's tile to the path-cost volume: lane l stores 16-byte
-    // pieces l, l+64, ...  A piece belongs to one tile row (a line of a
-    // horizontal path, a step of the vertical one), which is one contiguous run
-    // of the volume; pieces of absent lines or steps past the line get an
-    // out-of-range offset and are dropped.
-    auto flush = [&](int sc, int nSteps, bool full) {
-        const ChunkAddr<U> co = chunk(sc, volRow, VB, VB, full);
-        const auto rOut = rsrc(reinterpret_cast<char*>(out) + co.origin, co.nr);
-        constexpr int PPR = VERT ? LPW * VB / 16 : U * VB / 16;  // pieces per tile row
-        const int nRows = VERT ? nSteps : nValid;
-        const int nCols = VERT ? nValid : nSteps;                // pixels present in a row
+    // ---- compute wave --------------------------------------------------------
+    struct CBuf {
+        u4v q[4];
+        int w;
+    };
+    // This lane's costs of chunk k (and its row's per-step word) from LDS.
+    auto cread = [&](CBuf& c, int k) {
+        const char* const p = s_lds + (k % 3) * TL::CSLOT + lane * TL::LANE;
 #pragma unroll
-        for (int i = 0; i < TL::PIECES / 64; ++i) {
-            const int pc = lane + 64 * i;
-            const int row = pc / PPR;
-            const int ob = (pc - row * PPR) * 16;                 // byte offset inside the row
-            const int tbyte = row * (VERT ? LPW * VB : TL::ROWH) + ob;
-            typedef uint32_t u4 __attribute__((ext_vector_type(4)));
-            const u4 v = *reinterpret_cast<const u4*>(s_tile + tbyte);
-            const bool ok = row < nRows && ob / VB < nCols;
-            const uint32_t go = ok ? (uint32_t)row * volRow + (uint32_t)ob : kOffLimit;
-            __builtin_amdgcn_raw_buffer_store_b128(v, rOut, go, 0, 0);
-        }
+        for (int i = 0; i < 4; ++i) c.q[i] = *reinterpret_cast<const u4v*>(p + 16 * i);
+        if constexpr (WORDS)
+            c.w = *reinterpret_cast<const int*>(s_lds + TL::OFF_W + (k % 3) * TL::WSLOT +
+                                                (j * 16 + r16) * 4);
     };
 
-    // Process one prefetched chunk [sc, sc+U); full: all U steps exist.
-    auto process = [&](const StepBuf<K, U>& buf, int sc, bool full) {
+    // Process chunk k; full: all U steps exist.
+    auto process = [&](const CBuf& c, int k, bool full) {
+        const int sc = chunkStart(k);
         int p2v = 0, shv = 0;
         uint32_t moveMask = 0, bigMask = 0;
         if constexpr (REFINE) {
-            const int base = (int)(int16_t)(buf.w & 0xFFFF);
+            const int base = (int)(int16_t)(c.w & 0xFFFF);
             const int prevBase = dpp<kDppRowShr1>(carryBase, base);
             shv = base - prevBase;
             // Bit u: some line of the wave moves at step u (any shift / |shift| > 1).
@@ -590,18 +731,22 @@ __device__ __forceinline__ void runLine(const PathArgs& a, int lineGroup, const 
             carryBase = rowBcast<U - 1>(base);
         }
         if constexpr (ADAPT) {
-            const int im4 = REFINE ? (int)((uint32_t)buf.w >> 22) : buf.w << 2;
+            const int im4 = REFINE ? (int)((uint32_t)c.w >> 22) : c.w << 2;
             const int prevIm4 = dpp<kDppRowShr1>(carryIm4, im4);
             const uint32_t d4 = __builtin_amdgcn_sad_u16((uint32_t)im4, (uint32_t)prevIm4, 0u);
             p2v = *reinterpret_cast<const int*>(reinterpret_cast<const char*>(s_lut) + d4);
             carryIm4 = rowBcast<U - 1>(im4);
         }
         const int nSteps = full ? U : imin(U, s1 - sc);
+        char* const tile = s_lds + TL::OFF_T + (k & 1) * TL::TSLOT;
         auto run = [&](auto uc) {
             constexpr int u = decltype(uc)::value;
             if (full || u < nSteps) {
                 const int p2 = ADAPT ? rowBcast<u>(p2v) : a.P2;
                 const bool moved = REFINE && ((moveMask >> u) & 1u);
+                uint32_t cw[W32];
+#pragma unroll
+                for (int w = 0; w < W32; ++w) cw[w] = c.q[(u * W32 + w) >> 2][(u * W32 + w) & 3];
                 // Tile slot of step u: horizontal rows run in address order (R->L
                 // reversed), the vertical tile is step-major.
                 int tb;
@@ -609,60 +754,51 @@ __device__ __forceinline__ void runLine(const PathArgs& a, int lineGroup, const 
                     tb = (u * LPW + j) * VB + g * K * 2;
                 else
                     tb = j * TL::ROWH + (RL ? (full ? U - 1 - u : nSteps - 1 - u) : u) * VB + g * K * 2;
-                step(uc, buf.c[u], p2, moved, bigMask, shv, tb);
+                step(uc, cw, p2, moved, bigMask, shv, tile + tb);
             }
         };
         unrollSteps<U>(run);
     };
 
-    // Three register buffers in rotation: chunk k's loads are issued before
-    // chunk k-2 is processed, so two chunks of steps hide their latency, and
-    // each chunk's tile is written back right after its steps.  Triples of full
-    // chunks run with no conditional loads, stores or steps (a guarded path
-    // makes the compiler's wait counts drain the look-ahead: vmcnt retires
-    // loads and stores together, in issue order); the remaining steps follow
-    // one chunk at a time.
-    StepBuf<K, U> bufA, bufB, bufC;
-    int sc = s0;
-    issue(bufA, sc, sc + U <= s1);
-    issue(bufB, sc + U, sc + 2 * U <= s1);
-    for (; sc + 5 * U <= s1; sc += 3 * U) {
-        issue(bufC, sc + 2 * U, true);
-        process(bufA, sc, true);
-        flush(sc, U, true);
-        issue(bufA, sc + 3 * U, true);
-        process(bufB, sc + U, true);
-        flush(sc + U, U, true);
-        issue(bufB, sc + 4 * U, true);
-        process(bufC, sc + 2 * U, true);
-        flush(sc + 2 * U, U, true);
+    // Chunk k+1 is read from LDS while chunk k is processed; barrier k+1 ends
+    // chunk k (its tile complete, slot k free for the memory wave).
+    wgBarrier();
+    CBuf cA, cB;
+    cread(cA, 0);
+    int k = 0;
+    for (; k + 2 <= nFull; k += 2) {
+        cread(cB, k + 1);
+        process(cA, k, true);
+        wgBarrier();
+        if (k + 2 < nChunks) cread(cA, k + 2);
+        process(cB, k + 1, true);
+        wgBarrier();
     }
-    for (; sc < s1; sc += U) {
-        issue(bufC, sc + 2 * U, false);
-        process(bufA, sc, false);
-        flush(sc, imin(U, s1 - sc), false);
-        bufA = bufB;
-        bufB = bufC;
+    for (; k < nChunks; ++k) {
+        if (k + 1 < nChunks) cread(cB, k + 1);
+        process(cA, k, chunkFull(k));
+        wgBarrier();
+        cA = cB;
     }
 }
 
 template <int G, int K, int MODE, bool ADAPT, bool EXACT>
-__global__ __launch_bounds__(64) void kPaths(PathArgs a)
+__global__ __launch_bounds__(128) void kPaths(PathArgs a)
 {
     __shared__ int s_lut[256];
     __shared__ int s_xchg[MODE == kRefine ? 64 * K : 1];
-    __shared__ __attribute__((aligned(16))) char s_tile[Tile<G, K>::BYTES];
+    __shared__ __attribute__((aligned(16))) char s_lds[Lds<G, K>::BYTES];
     if constexpr (ADAPT) {
-        for (int i = threadIdx.x; i < 256; i += 64) s_lut[i] = a.lut[i];
+        for (int i = threadIdx.x; i < 256; i += 128) s_lut[i] = a.lut[i];
         __syncthreads();
     }
     const int b = blockIdx.x;
     if (b < a.nLRBlocks)
-        runLine<G, K, MODE, ADAPT, EXACT, kLR>(a, b, s_lut, s_xchg, s_tile);
+        runLine<G, K, MODE, ADAPT, EXACT, kLR>(a, b, s_lut, s_xchg, s_lds);
     else if (b < a.nLRBlocks + a.nRLBlocks)
-        runLine<G, K, MODE, ADAPT, EXACT, kRL>(a, b - a.nLRBlocks, s_lut, s_xchg, s_tile);
+        runLine<G, K, MODE, ADAPT, EXACT, kRL>(a, b - a.nLRBlocks, s_lut, s_xchg, s_lds);
     else
-        runLine<G, K, MODE, ADAPT, EXACT, kTB>(a, b - a.nLRBlocks - a.nRLBlocks, s_lut, s_xchg, s_tile);
+        runLine<G, K, MODE, ADAPT, EXACT, kTB>(a, b - a.nLRBlocks - a.nRLBlocks, s_lut, s_xchg, s_lds);
 }
 
 inline int linesPerWave(int Dp) { return Dp <= 16 ? 4 : (Dp <= 32 ? 2 : 1); }
@@ -677,13 +813,13 @@ hipError_t dispatchPaths(const PathArgs& a, hipStream_t s)
     const int nTB = a.numPaths >= 3 ? (a.W + lpw - 1) / lpw : 0;
     const int blocks = b.nLRBlocks + b.nRLBlocks + nTB;
     switch (a.Dp) {
-    case 16: kPaths<16, 1, MODE, ADAPT, EXACT><<<blocks, 64, 0, s>>>(b); break;
-    case 32: kPaths<32, 1, MODE, ADAPT, EXACT><<<blocks, 64, 0, s>>>(b); break;
-    case 64: kPaths<64, 1, MODE, ADAPT, EXACT><<<blocks, 64, 0, s>>>(b); break;
-    case 128: kPaths<64, 2, MODE, ADAPT, EXACT><<<blocks, 64, 0, s>>>(b); break;
-    case 256: kPaths<64, 4, MODE, ADAPT, EXACT><<<blocks, 64, 0, s>>>(b); break;
-    case 512: kPaths<64, 8, MODE, ADAPT, EXACT><<<blocks, 64, 0, s>>>(b); break;
-    case 1024: kPaths<64, 16, MODE, ADAPT, EXACT><<<blocks, 64, 0, s>>>(b); break;
+    case 16: kPaths<16, 1, MODE, ADAPT, EXACT><<<blocks, 128, 0, s>>>(b); break;
+    case 32: kPaths<32, 1, MODE, ADAPT, EXACT><<<blocks, 128, 0, s>>>(b); break;
+    case 64: kPaths<64, 1, MODE, ADAPT, EXACT><<<blocks, 128, 0, s>>>(b); break;
+    case 128: kPaths<64, 2, MODE, ADAPT, EXACT><<<blocks, 128, 0, s>>>(b); break;
+    case 256: kPaths<64, 4, MODE, ADAPT, EXACT><<<blocks, 128, 0, s>>>(b); break;
+    case 512: kPaths<64, 8, MODE, ADAPT, EXACT><<<blocks, 128, 0, s>>>(b); break;
+    case 1024: kPaths<64, 16, MODE, ADAPT, EXACT><<<blocks, 128, 0, s>>>(b); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -226,3 +226,31 @@ def test_matcher_parity(w, h, D, levels, cost, adaptive, median):
     U.assert_same("disp16", d16, want.disp16)
     U.assert_same("disp32", d32, want.disp32)
     U.assert_same("conf", cf, want.conf)
+
+
+@pytest.mark.parametrize("kind", ["single", "coarse", "refine"])
+def test_census_16bit_volumes(ws, kind):
+    """Census with P2 = 300: Cmax + max(P2') > 254, so the volumes stay 16-bit
+    (the byte form is only used when every path cost fits in 8 bits)."""
+    w, h = 160, 72
+    left, right = pair(w, h, 60, seed=91)
+    if kind == "single":
+        kw = dict(block=5, P1=20, P2=300, cost=1, post=3, paths=3, adaptive=1)
+        rc, d16, d32, cf = U.single_scale(ws, left, right, 64, **kw)
+        want = O.compute_single_scale(left, right, 64, **kw)
+    elif kind == "coarse":
+        kw = dict(block=3, P1=20, P2=300, cost=1, post=1, paths=3, adaptive=1)
+        rc, d16, d32, cf = U.level(ws, left, right, 64, 63, **kw)
+        want = O.compute_level(left, right, 64, 63, **kw)
+    else:
+        pl, pr = O.pyrdown(left), O.pyrdown(right)
+        parent = O.compute_level(pl, pr, 32, 31, block=3, P1=20, P2=300, cost=1, post=1,
+                                 want32=False)
+        kw = dict(block=3, P1=20, P2=300, cost=1, post=3, paths=3, adaptive=1, refine=True,
+                  min_conf=0.05)
+        rc, d16, d32, cf = U.level(ws, left, right, 16, 63, parent.disp16, parent.conf, **kw)
+        want = O.compute_level(left, right, 16, 63, parent.disp16, parent.conf, **kw)
+    assert rc == 0 and want.rc == 0
+    U.assert_same("disp16", d16, want.disp16)
+    U.assert_same("disp32", d32, want.disp32)
+    U.assert_same("conf", cf, want.conf)

@@ -136,6 +136,15 @@ def lib() -> C.CDLL:
         L.pyrsgm_matcher_device_bytes.argtypes = [vp]
         L.pyrsgm_matcher_device_bytes.restype = sz
         L.pyrsgm_version.restype = C.c_char_p
+        L.pyrsgm_batch_create.argtypes = [ci, ci, ci, C.POINTER(_ParamsC), C.c_char_p, ci]
+        L.pyrsgm_batch_create.restype = vp
+        L.pyrsgm_batch_destroy.argtypes = [vp]
+        L.pyrsgm_batch_destroy.restype = None
+        L.pyrsgm_batch_compute.argtypes = [vp] + [vp, sz, sz] * 5 + [vp, C.c_char_p, ci]
+        L.pyrsgm_batch_capture.argtypes = [vp] + [vp, sz, sz] * 5 + [C.c_char_p, ci]
+        L.pyrsgm_batch_launch.argtypes = [vp, vp, C.c_char_p, ci]
+        L.pyrsgm_batch_device_bytes.argtypes = [vp]
+        L.pyrsgm_batch_device_bytes.restype = sz
         _lib = L
     return _lib
 
@@ -282,3 +291,74 @@ class StereoMatcher:
 
     def device_memory_bytes(self) -> int:
         return lib().pyrsgm_matcher_device_bytes(self._h)
+
+
+def _bptr(t):
+    """(pointer, row pitch, pair stride) of a [N, H, W] device tensor (or None)."""
+    if t is None:
+        return None, 0, 0
+    es = t.element_size()
+    return t.data_ptr(), t.stride(1) * es, t.stride(0) * es
+
+
+class BatchMatcher:
+    """pyrsgm::BatchMatcher: N pairs of one size per launch (include/pyrsgm/pyrsgm_batch.h).
+
+    Tensors are [N, H, W] device tensors (uint8 in, int16 / float32 out), each
+    pair's rows contiguous up to the row stride."""
+
+    def __init__(self, width: int, height: int, batch: int, params: Params):
+        err = C.create_string_buffer(512)
+        h = lib().pyrsgm_batch_create(width, height, batch, C.byref(_to_c(params)), err, 512)
+        if not h:
+            msg = err.value.decode()
+            raise (ValueError if "pyrsgm:" in msg and "HIP" not in msg else RuntimeError)(msg)
+        self._h = h
+        self.width, self.height, self.batch = width, height, batch
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pyrsgm_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _args(self, left, right, disp16, disp32, conf):
+        for t in (left, right, disp16, disp32, conf):
+            if t is not None and (t.dim() != 3 or t.shape[0] != self.batch or
+                                  t.shape[1] != self.height or t.shape[2] != self.width):
+                raise ValueError(f"pyrsgm: expected [{self.batch}, {self.height}, {self.width}] "
+                                 f"tensors, got {tuple(t.shape)}")
+        out = []
+        for t in (left, right, disp16, disp32, conf):
+            out += list(_bptr(t))
+        return out
+
+    def compute(self, left, right, disp16=None, disp32=None, conf=None, stream=None):
+        import torch
+        s = stream if stream is not None else torch.cuda.current_stream()
+        err = C.create_string_buffer(512)
+        if lib().pyrsgm_batch_compute(self._h, *self._args(left, right, disp16, disp32, conf),
+                                      s.cuda_stream, err, 512) != 0:
+            raise RuntimeError(err.value.decode())
+
+    def capture(self, left, right, disp16=None, disp32=None, conf=None):
+        """Record compute() on these tensors into a HIP graph; replay with launch()."""
+        err = C.create_string_buffer(512)
+        if lib().pyrsgm_batch_capture(self._h, *self._args(left, right, disp16, disp32, conf),
+                                      err, 512) != 0:
+            raise RuntimeError(err.value.decode())
+
+    def launch(self, stream=None):
+        import torch
+        s = stream if stream is not None else torch.cuda.current_stream()
+        err = C.create_string_buffer(512)
+        if lib().pyrsgm_batch_launch(self._h, s.cuda_stream, err, 512) != 0:
+            raise RuntimeError(err.value.decode())
+
+    def device_memory_bytes(self) -> int:
+        return lib().pyrsgm_batch_device_bytes(self._h)

@@ -57,20 +57,22 @@ void releaseWorkspace(Workspace* ws)
     *ws = Workspace{};
 }
 
-int ensureWorkspace(Workspace* ws, int W, int H, int D)
+int ensureWorkspace(Workspace* ws, int W, int H, int D, int n)
 {
-    if (!ws) return -1;
-    if (ws->base && ws->W == W && ws->H == H && ws->D == D) return 0;
+    if (!ws || n < 1) return -1;
+    if (ws->base && ws->W == W && ws->H == H && ws->D == D && ws->n == n) return 0;
     releaseWorkspace(ws);
     const Layout l = layoutFor(W, H, D);
     void* base = nullptr;
-    if (hipMalloc(&base, l.total) != hipSuccess) {
+    if (hipMalloc(&base, l.total * (size_t)n) != hipSuccess) {
         (void)hipGetLastError();
         return -1;
     }
     char* b = static_cast<char*>(base);
     ws->base = base;
-    ws->bytes = l.total;
+    ws->bytes = l.total * (size_t)n;
+    ws->n = n;
+    ws->slice = l.total;
     ws->W = W;
     ws->H = H;
     ws->D = D;
@@ -180,6 +182,7 @@ hipError_t launchLevel(const LevelCall& c, Workspace& ws, hipStream_t s)
     a.keys = ws.keys;
     a.uniq = c.uniq;
     a.minConf = c.minConf;
+    a.batch = c.batch;
     memcpy(a.lut, c.lut, sizeof(a.lut));
 
     profBegin(PYRSGM_STAGE_PATHS, c.W, c.H, c.D, s);
@@ -222,42 +225,23 @@ bool paramsOk(int block, int P1, int P2)
 
 } // namespace
 
-extern "C" {
+namespace pyr {
 
-void* pyrsgm_create_workspace(void)
-{
-    return new (std::nothrow) Workspace();
-}
-
-void pyrsgm_destroy_workspace(void* workspace)
-{
-    Workspace* ws = static_cast<Workspace*>(workspace);
-    if (!ws) return;
-    releaseWorkspace(ws);
-    delete ws;
-}
-
-int pyrsgm_workspace_reserve(void* workspace, int width, int height, int numDisparities)
-{
-    if (!workspace || !geometryOk(width, height, numDisparities)) return -1;
-    return ensureWorkspace(static_cast<Workspace*>(workspace), width, height, numDisparities);
-}
-
-// Reference: sgm_kernels.cu:1864-2076.
-int pyrsgm_compute_single_scale(
+// Reference: sgm_kernels.cu:1864-2076 (b.n pairs).
+int computeSingleScale(
     void* workspace, const unsigned char* d_left_gray, const unsigned char* d_right_gray,
     short* d_disp, float* d_disp_float, float* d_conf_float, size_t left_step_bytes,
     size_t right_step_bytes, size_t disp_step_bytes, size_t disp_float_step_bytes,
     size_t conf_float_step_bytes, int width, int height, int numDisparities, int minDisparity,
     int blockSize, int P1, int P2, int uniquenessRatio, int disp12MaxDiff, int preFilterCap,
     int costMode, int postFlags, int numPaths, int adaptiveP2, float p2Alpha, float p2Beta,
-    float p2LowerMult, float p2UpperMult, hipStream_t stream)
+    float p2LowerMult, float p2UpperMult, const Batch& b, hipStream_t stream)
 {
     if (!workspace || !d_left_gray || !d_right_gray || !geometryOk(width, height, numDisparities) ||
         !paramsOk(blockSize, P1, P2))
         return -1;
     Workspace* ws = static_cast<Workspace*>(workspace);
-    if (ensureWorkspace(ws, width, height, numDisparities) != 0) {
+    if (ensureWorkspace(ws, width, height, numDisparities, b.n) != 0) {
         fprintf(stderr, "pyrsgm: workspace allocation failed\n");
         return -1;
     }
@@ -290,12 +274,14 @@ int pyrsgm_compute_single_scale(
     c.numPaths = std::min(3, std::max(1, numPaths));
     c.mode = kSingle;
     c.adaptive = buildP2Lut(P2, adaptiveP2, p2Alpha, p2Beta, p2LowerMult, p2UpperMult, c.lut);
+    c.batch = b;
+    c.batch.ws = ws->slice;
     const hipError_t e = launchLevel(c, *ws, stream);
     return e == hipSuccess ? 0 : fail(e, "pyrsgm_compute_single_scale");
 }
 
-// Reference: sgm_kernels.cu:1617-1862.
-int pyrsgm_compute_level(
+// Reference: sgm_kernels.cu:1617-1862 (b.n pairs).
+int computeLevel(
     void* workspace, const unsigned char* d_left_gray, const unsigned char* d_right_gray,
     const short* d_parent_disp, const float* d_parent_conf, short* d_disp, float* d_disp_float,
     float* d_conf_float, size_t left_step_bytes, size_t right_step_bytes,
@@ -305,7 +291,7 @@ int pyrsgm_compute_level(
     int maxDisparity, int blockSize, int P1, int P2, int uniquenessRatio, int disp12MaxDiff,
     int preFilterCap, int costMode, int postFlags, int numPaths, int refineLevel,
     float minConfidence, int adaptiveP2, float p2Alpha, float p2Beta, float p2LowerMult,
-    float p2UpperMult, hipStream_t stream)
+    float p2UpperMult, const Batch& b, hipStream_t stream)
 {
     if (!workspace || !d_left_gray || !d_right_gray || !geometryOk(width, height, numDisparities) ||
         (refineLevel && numDisparities > 256) || !paramsOk(blockSize, P1, P2))
@@ -317,7 +303,7 @@ int pyrsgm_compute_level(
     if (!refineLevel) maxDisparity = std::min(maxDisparity, minDisparity + numDisparities - 1);
     if (maxDisparity < minDisparity) return -1;
     Workspace* ws = static_cast<Workspace*>(workspace);
-    if (ensureWorkspace(ws, width, height, numDisparities) != 0) {
+    if (ensureWorkspace(ws, width, height, numDisparities, b.n) != 0) {
         fprintf(stderr, "pyrsgm: workspace allocation failed\n");
         return -1;
     }
@@ -357,8 +343,75 @@ int pyrsgm_compute_level(
     c.mode = refineLevel ? kRefine : kCoarse;
     c.minConf = fminf(fmaxf(minConfidence, 0.0f), 1.0f);
     c.adaptive = buildP2Lut(P2, adaptiveP2, p2Alpha, p2Beta, p2LowerMult, p2UpperMult, c.lut);
+    c.batch = b;
+    c.batch.ws = ws->slice;
     const hipError_t e = launchLevel(c, *ws, stream);
     return e == hipSuccess ? 0 : fail(e, "pyrsgm_compute_level");
+}
+
+} // namespace pyr
+
+extern "C" {
+
+void* pyrsgm_create_workspace(void)
+{
+    return new (std::nothrow) Workspace();
+}
+
+void pyrsgm_destroy_workspace(void* workspace)
+{
+    Workspace* ws = static_cast<Workspace*>(workspace);
+    if (!ws) return;
+    releaseWorkspace(ws);
+    delete ws;
+}
+
+int pyrsgm_workspace_reserve(void* workspace, int width, int height, int numDisparities)
+{
+    if (!workspace || !geometryOk(width, height, numDisparities)) return -1;
+    return ensureWorkspace(static_cast<Workspace*>(workspace), width, height, numDisparities);
+}
+
+// Reference: sgm_kernels.cu:1864-2076.
+int pyrsgm_compute_single_scale(
+    void* workspace, const unsigned char* d_left_gray, const unsigned char* d_right_gray,
+    short* d_disp, float* d_disp_float, float* d_conf_float, size_t left_step_bytes,
+    size_t right_step_bytes, size_t disp_step_bytes, size_t disp_float_step_bytes,
+    size_t conf_float_step_bytes, int width, int height, int numDisparities, int minDisparity,
+    int blockSize, int P1, int P2, int uniquenessRatio, int disp12MaxDiff, int preFilterCap,
+    int costMode, int postFlags, int numPaths, int adaptiveP2, float p2Alpha, float p2Beta,
+    float p2LowerMult, float p2UpperMult, hipStream_t stream)
+{
+    return computeSingleScale(workspace, d_left_gray, d_right_gray, d_disp, d_disp_float,
+                              d_conf_float, left_step_bytes, right_step_bytes, disp_step_bytes,
+                              disp_float_step_bytes, conf_float_step_bytes, width, height,
+                              numDisparities, minDisparity, blockSize, P1, P2, uniquenessRatio,
+                              disp12MaxDiff, preFilterCap, costMode, postFlags, numPaths,
+                              adaptiveP2, p2Alpha, p2Beta, p2LowerMult, p2UpperMult, Batch{},
+                              stream);
+}
+
+// Reference: sgm_kernels.cu:1617-1862.
+int pyrsgm_compute_level(
+    void* workspace, const unsigned char* d_left_gray, const unsigned char* d_right_gray,
+    const short* d_parent_disp, const float* d_parent_conf, short* d_disp, float* d_disp_float,
+    float* d_conf_float, size_t left_step_bytes, size_t right_step_bytes,
+    size_t parent_disp_step_bytes, size_t parent_conf_step_bytes, size_t disp_step_bytes,
+    size_t disp_float_step_bytes, size_t conf_float_step_bytes, int width, int height,
+    int parent_width, int parent_height, int numDisparities, int minDisparity,
+    int maxDisparity, int blockSize, int P1, int P2, int uniquenessRatio, int disp12MaxDiff,
+    int preFilterCap, int costMode, int postFlags, int numPaths, int refineLevel,
+    float minConfidence, int adaptiveP2, float p2Alpha, float p2Beta, float p2LowerMult,
+    float p2UpperMult, hipStream_t stream)
+{
+    return computeLevel(workspace, d_left_gray, d_right_gray, d_parent_disp, d_parent_conf,
+                        d_disp, d_disp_float, d_conf_float, left_step_bytes, right_step_bytes,
+                        parent_disp_step_bytes, parent_conf_step_bytes, disp_step_bytes,
+                        disp_float_step_bytes, conf_float_step_bytes, width, height,
+                        parent_width, parent_height, numDisparities, minDisparity, maxDisparity,
+                        blockSize, P1, P2, uniquenessRatio, disp12MaxDiff, preFilterCap, costMode,
+                        postFlags, numPaths, refineLevel, minConfidence, adaptiveP2, p2Alpha,
+                        p2Beta, p2LowerMult, p2UpperMult, Batch{}, stream);
 }
 
 // Reference: pyrdown.cu:56-74.

@@ -20,9 +20,29 @@ inline int paddedLabels(int D)
     return p;
 }
 
-// Device workspace of one (W, H, D) geometry.  One allocation, carved up.
+// Pairs of one batched launch.  Pair p's buffers are pair 0's offset by p times
+// these strides (bytes); a per-pair call has n = 1 and zero strides.  Kernels
+// take the pair from a grid dimension.
+struct Batch {
+    int n = 1;
+    size_t ws = 0;                // workspace slice
+    size_t left = 0, right = 0;   // input images
+    size_t pdisp = 0, pconf = 0;  // parent level's results (refinement)
+    size_t out16 = 0, out32 = 0, outc = 0;
+};
+
+template <class T>
+__host__ __device__ __forceinline__ T* pairAt(T* p, size_t pair, size_t stride)
+{
+    return p ? reinterpret_cast<T*>(reinterpret_cast<uintptr_t>(p) + pair * stride) : p;
+}
+
+// Device workspace of one (W, H, D) geometry for n pairs.  One allocation,
+// carved up; the pointers address pair 0's slice.
 struct Workspace {
     int W = 0, H = 0, D = 0, Dp = 0;
+    int n = 0;                    // pair slices
+    size_t slice = 0;             // bytes per slice
     void* base = nullptr;
     size_t bytes = 0;
     uint2* desc = nullptr;      // [2][P] census words (x) or packed BT intervals (x: grad, y: intensity)
@@ -37,7 +57,8 @@ struct Workspace {
 };
 
 size_t workspaceBytes(int W, int H, int D);
-int ensureWorkspace(Workspace* ws, int W, int H, int D);
+// (Re)allocates n slices of one level's buffers unless the geometry matches.
+int ensureWorkspace(Workspace* ws, int W, int H, int D, int n = 1);
 void releaseWorkspace(Workspace* ws);
 
 // Everything one level needs; filled by the C-ABI entry points.
@@ -52,6 +73,7 @@ struct LevelCall {
     int mode;        // Mode
     float minConf;
     int adaptive;
+    Batch batch;
     int32_t lut[256];
 };
 
@@ -91,6 +113,7 @@ struct PathArgs {
     uint32_t* keys;
     int uniq;
     float minConf;
+    Batch batch;
     int32_t lut[256];
 };
 
@@ -99,6 +122,35 @@ hipError_t launchPathsSingle(const PathArgs& a, int adaptive, hipStream_t s);
 hipError_t launchPathsCoarse(const PathArgs& a, int adaptive, hipStream_t s);
 hipError_t launchPathsRefine(const PathArgs& a, int adaptive, hipStream_t s);
 hipError_t launchWTA(const PathArgs& a, int mode, hipStream_t s);
+
+// The C-ABI level / single-scale computations for b.n pairs at once (the
+// extern "C" entry points call these with one pair).
+int computeSingleScale(void* workspace, const unsigned char* d_left_gray,
+                       const unsigned char* d_right_gray, short* d_disp, float* d_disp_float,
+                       float* d_conf_float, size_t left_step_bytes, size_t right_step_bytes,
+                       size_t disp_step_bytes, size_t disp_float_step_bytes,
+                       size_t conf_float_step_bytes, int width, int height, int numDisparities,
+                       int minDisparity, int blockSize, int P1, int P2, int uniquenessRatio,
+                       int disp12MaxDiff, int preFilterCap, int costMode, int postFlags,
+                       int numPaths, int adaptiveP2, float p2Alpha, float p2Beta,
+                       float p2LowerMult, float p2UpperMult, const Batch& b, hipStream_t stream);
+int computeLevel(void* workspace, const unsigned char* d_left_gray,
+                 const unsigned char* d_right_gray, const short* d_parent_disp,
+                 const float* d_parent_conf, short* d_disp, float* d_disp_float,
+                 float* d_conf_float, size_t left_step_bytes, size_t right_step_bytes,
+                 size_t parent_disp_step_bytes, size_t parent_conf_step_bytes,
+                 size_t disp_step_bytes, size_t disp_float_step_bytes,
+                 size_t conf_float_step_bytes, int width, int height, int parent_width,
+                 int parent_height, int numDisparities, int minDisparity, int maxDisparity,
+                 int blockSize, int P1, int P2, int uniquenessRatio, int disp12MaxDiff,
+                 int preFilterCap, int costMode, int postFlags, int numPaths, int refineLevel,
+                 float minConfidence, int adaptiveP2, float p2Alpha, float p2Beta,
+                 float p2LowerMult, float p2UpperMult, const Batch& b, hipStream_t stream);
+// pyrDown of n pairs of (one or two) images; src / dst pair strides in bytes.
+hipError_t launchPyrDownBatch(const uint8_t* srcA, int sstepA, const uint8_t* srcB, int sstepB,
+                              size_t sStride, int sw, int sh, uint8_t* dstA, int dstepA,
+                              uint8_t* dstB, int dstepB, size_t dStride, int dw, int dh, int n,
+                              hipStream_t stream);
 
 hipError_t launchPrep(const LevelCall& c, Workspace& ws, hipStream_t s);
 hipError_t launchCost(const LevelCall& c, Workspace& ws, hipStream_t s);

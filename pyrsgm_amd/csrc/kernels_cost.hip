@@ -57,12 +57,17 @@ __device__ __forceinline__ int btGrad(const uint8_t* img, int step, int W, int H
 // ---- descriptors + projection-key reset ----------------------------------
 __global__ __launch_bounds__(256) void kPrep(const uint8_t* left, int lstep, const uint8_t* right,
                                              int rstep, uint2* desc, uint32_t* keys, int W, int H,
-                                             int census, int cap)
+                                             int census, int cap, Batch b)
 {
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int z = blockIdx.z;
+    const int z = blockIdx.z & 1;          // view
+    const size_t pr = blockIdx.z >> 1;     // pair
     if (x >= W || y >= H) return;
+    left = pairAt(left, pr, b.left);
+    right = pairAt(right, pr, b.right);
+    desc = pairAt(desc, pr, b.ws);
+    keys = pairAt(keys, pr, b.ws);
     const uint8_t* img = z ? right : left;
     const int step = z ? rstep : lstep;
     const size_t p = (size_t)y * W + x;
@@ -119,9 +124,12 @@ __device__ __forceinline__ void storeCost8(int16_t* dst, const int (&v)[8])
 // SHRT_MAX: that is exactly the reference's labelValid = geometry && C < MAX
 // (sgm_kernels.cu:1135-1143), so the path kernels test "C < SHRT_MAX" alone.
 __global__ __launch_bounds__(256) void kCensusCost(const uint2* desc, int16_t* C, int W, int H,
-                                                   int D, int Dp, int minD, int maxD, int geomMask)
+                                                   int D, int Dp, int minD, int maxD, int geomMask,
+                                                   size_t wsStride)
 {
     extern __shared__ uint32_t s_r[];
+    desc = pairAt(desc, blockIdx.z, wsStride);
+    C = pairAt(C, blockIdx.z, wsStride);
     const int tpp = Dp >> 3;
     const int npx = 256 / tpp;
     const int y = blockIdx.y;
@@ -194,8 +202,16 @@ __device__ __forceinline__ int32_t labelWord(int base, int valid, uint8_t im)
 // ---- Census cost at refinement levels (sgm_kernels.cu:1112-1133) ------------
 __global__ __launch_bounds__(256) void kCensusLocal(const uint2* desc, RefineIn r, int32_t* labels,
                                                     const uint8_t* img, int istep, int16_t* C,
-                                                    int W, int H, int D, int Dp, int minD, int maxD)
+                                                    int W, int H, int D, int Dp, int minD, int maxD,
+                                                    Batch b)
 {
+    const size_t pr = blockIdx.z;
+    desc = pairAt(desc, pr, b.ws);
+    labels = pairAt(labels, pr, b.ws);
+    C = pairAt(C, pr, b.ws);
+    img = pairAt(img, pr, b.left);
+    r.pdisp = pairAt(r.pdisp, pr, b.pdisp);
+    r.pconf = pairAt(r.pconf, pr, b.pconf);
     const int tpp = Dp >> 3;
     const int npx = 256 / tpp;
     const int y = blockIdx.y;
@@ -229,13 +245,17 @@ constexpr int kSegX = 64;
 constexpr int kChunkD = 64;
 
 __global__ __launch_bounds__(256) void kBTRow(const uint2* desc, int16_t* Hs, int W, int H,
-                                              int Dp, int minD, int half)
+                                              int Dp, int minD, int half, size_t wsStride)
 {
     extern __shared__ uint8_t s_bt[];
     const int dc = imin(Dp, kChunkD);
+    const int nzc = Dp / dc;  // label chunks per pair
+    const size_t pr = blockIdx.z / nzc;
+    desc = pairAt(desc, pr, wsStride);
+    Hs = pairAt(Hs, pr, wsStride);
     const int y = blockIdx.y;
     const int x0 = blockIdx.x * kSegX;
-    const int dc0 = blockIdx.z * dc;
+    const int dc0 = (blockIdx.z - (int)pr * nzc) * dc;
     const int ext = kSegX + 2 * half;
     const uint2* L = desc + (size_t)y * W;
     const uint2* R = desc + (size_t)W * H + (size_t)y * W;
@@ -269,8 +289,11 @@ __global__ __launch_bounds__(256) void kBTRow(const uint2* desc, int16_t* Hs, in
 
 // ---- vertical window sum (sgm_kernels.cu:390-416) ----------------------------
 __global__ __launch_bounds__(256) void kBoxV(const int16_t* Hs, int16_t* C, int W, int H, int D,
-                                             int Dp, int half, int minD, int maxD, int geomMask)
+                                             int Dp, int half, int minD, int maxD, int geomMask,
+                                             size_t wsStride)
 {
+    Hs = pairAt(Hs, blockIdx.y, wsStride);
+    C = pairAt(C, blockIdx.y, wsStride);
     const size_t oct = (size_t)blockIdx.x * 256 + threadIdx.x;
     const size_t nOct = (size_t)W * H * (Dp >> 3);
     if (oct >= nOct) return;
@@ -304,8 +327,16 @@ __global__ __launch_bounds__(256) void kBoxV(const int16_t* Hs, int16_t* C, int 
 // ---- BT window cost at refinement levels (sgm_kernels.cu:1075-1110) ----------
 __global__ __launch_bounds__(256) void kBTLocal(const uint2* desc, RefineIn r, int32_t* labels,
                                                 const uint8_t* img, int istep, int16_t* C, int W,
-                                                int H, int D, int Dp, int minD, int maxD, int half)
+                                                int H, int D, int Dp, int minD, int maxD, int half,
+                                                Batch b)
 {
+    const size_t pr = blockIdx.z;
+    desc = pairAt(desc, pr, b.ws);
+    labels = pairAt(labels, pr, b.ws);
+    C = pairAt(C, pr, b.ws);
+    img = pairAt(img, pr, b.left);
+    r.pdisp = pairAt(r.pdisp, pr, b.pdisp);
+    r.pconf = pairAt(r.pconf, pr, b.pconf);
     const int tpp = Dp >> 3;
     const int npx = 256 / tpp;
     const int y = blockIdx.y;
@@ -357,15 +388,17 @@ __device__ __forceinline__ int reflect101(int i, int n)
 }
 
 __global__ __launch_bounds__(256) void kPyrDown(const uint8_t* srcA, int sstepA, const uint8_t* srcB,
-                                                int sstepB, int sw, int sh, uint8_t* dstA,
-                                                int dstepA, uint8_t* dstB, int dstepB, int dw,
-                                                int dh)
+                                                int sstepB, size_t sStride, int sw, int sh,
+                                                uint8_t* dstA, int dstepA, uint8_t* dstB,
+                                                int dstepB, size_t dStride, int dw, int dh, int nv)
 {
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (x >= dw || y >= dh) return;
-    const uint8_t* src = blockIdx.z ? srcB : srcA;
-    const int sstep = blockIdx.z ? sstepB : sstepA;
+    const size_t pr = blockIdx.z / nv;              // pair
+    const int view = (int)(blockIdx.z - pr * nv);   // 0: A, 1: B
+    const uint8_t* src = pairAt(view ? srcB : srcA, pr, sStride);
+    const int sstep = view ? sstepB : sstepA;
     int cols[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) cols[k] = reflect101(2 * x + k - 2, sw);
@@ -378,8 +411,8 @@ __global__ __launch_bounds__(256) void kPyrDown(const uint8_t* srcA, int sstepA,
                        row[cols[4]];
         sum += w[ky] * rs;
     }
-    uint8_t* dst = blockIdx.z ? dstB : dstA;
-    const int dstep = blockIdx.z ? dstepB : dstepA;
+    uint8_t* dst = pairAt(view ? dstB : dstA, pr, dStride);
+    const int dstep = view ? dstepB : dstepA;
     dst[(size_t)y * dstep + x] = (uint8_t)((sum + 128) >> 8);
 }
 
@@ -387,9 +420,9 @@ __global__ __launch_bounds__(256) void kPyrDown(const uint8_t* srcA, int sstepA,
 
 hipError_t launchPrep(const LevelCall& c, Workspace& ws, hipStream_t s)
 {
-    dim3 grid((c.W + 63) / 64, (c.H + 3) / 4, 2);
+    dim3 grid((c.W + 63) / 64, (c.H + 3) / 4, 2 * c.batch.n);
     kPrep<<<grid, 256, 0, s>>>(c.left, c.lstep, c.right, c.rstep, ws.desc, ws.keys, c.W, c.H,
-                               c.cost == 1, c.cap);
+                               c.cost == 1, c.cap, c.batch);
     return hipGetLastError();
 }
 
@@ -400,31 +433,44 @@ hipError_t launchCost(const LevelCall& c, Workspace& ws, hipStream_t s)
     const int half = c.block / 2;
     if (c.mode == kRefine) {
         RefineIn r{c.pdisp, c.pconf, c.pdpitch, c.pcpitch, c.PW, c.PH, c.minConf};
-        dim3 grid((c.W + npx - 1) / npx, c.H);
+        dim3 grid((c.W + npx - 1) / npx, c.H, c.batch.n);
         if (c.cost == 1)
             kCensusLocal<<<grid, 256, 0, s>>>(ws.desc, r, ws.labels, c.left, c.lstep, ws.C, c.W,
-                                              c.H, c.D, c.Dp, c.minD, c.maxD);
+                                              c.H, c.D, c.Dp, c.minD, c.maxD, c.batch);
         else
             kBTLocal<<<grid, 256, 0, s>>>(ws.desc, r, ws.labels, c.left, c.lstep, ws.C, c.W, c.H,
-                                          c.D, c.Dp, c.minD, c.maxD, half);
+                                          c.D, c.Dp, c.minD, c.maxD, half, c.batch);
         return hipGetLastError();
     }
     if (c.cost == 1) {
-        dim3 grid((c.W + npx - 1) / npx, c.H);
+        dim3 grid((c.W + npx - 1) / npx, c.H, c.batch.n);
         const size_t lds = (size_t)(npx + c.Dp - 1) * sizeof(uint32_t);
         kCensusCost<<<grid, 256, lds, s>>>(ws.desc, ws.C, c.W, c.H, c.D, c.Dp, c.minD, c.maxD,
-                                           c.mode == kCoarse);
+                                           c.mode == kCoarse, c.batch.ws);
         return hipGetLastError();
     }
     const int dc = c.Dp < kChunkD ? c.Dp : kChunkD;
-    dim3 gridH((c.W + kSegX - 1) / kSegX, c.H, c.Dp / dc);
+    dim3 gridH((c.W + kSegX - 1) / kSegX, c.H, (c.Dp / dc) * c.batch.n);
     const size_t lds = (size_t)(kSegX + 2 * half) * dc;
-    kBTRow<<<gridH, 256, lds, s>>>(ws.desc, ws.Alr, c.W, c.H, c.Dp, c.minD, half);
+    kBTRow<<<gridH, 256, lds, s>>>(ws.desc, ws.Alr, c.W, c.H, c.Dp, c.minD, half, c.batch.ws);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const size_t nOct = (size_t)c.W * c.H * (c.Dp / 8);
-    kBoxV<<<(unsigned)((nOct + 255) / 256), 256, 0, s>>>(ws.Alr, ws.C, c.W, c.H, c.D, c.Dp, half,
-                                                          c.minD, c.maxD, c.mode == kCoarse);
+    dim3 gridV((unsigned)((nOct + 255) / 256), c.batch.n);
+    kBoxV<<<gridV, 256, 0, s>>>(ws.Alr, ws.C, c.W, c.H, c.D, c.Dp, half, c.minD, c.maxD,
+                                c.mode == kCoarse, c.batch.ws);
+    return hipGetLastError();
+}
+
+hipError_t launchPyrDownBatch(const uint8_t* srcA, int sstepA, const uint8_t* srcB, int sstepB,
+                              size_t sStride, int sw, int sh, uint8_t* dstA, int dstepA,
+                              uint8_t* dstB, int dstepB, size_t dStride, int dw, int dh, int n,
+                              hipStream_t stream)
+{
+    const int nv = srcB ? 2 : 1;
+    dim3 grid((dw + 63) / 64, (dh + 3) / 4, nv * n);
+    kPyrDown<<<grid, 256, 0, stream>>>(srcA, sstepA, srcB, sstepB, sStride, sw, sh, dstA, dstepA,
+                                       dstB, dstepB, dStride, dw, dh, nv);
     return hipGetLastError();
 }
 
@@ -432,10 +478,8 @@ hipError_t launchPyrDown2(const uint8_t* srcA, int sstepA, const uint8_t* srcB, 
                           int sw, int sh, uint8_t* dstA, int dstepA, uint8_t* dstB, int dstepB,
                           int dw, int dh, hipStream_t stream)
 {
-    dim3 grid((dw + 63) / 64, (dh + 3) / 4, srcB ? 2 : 1);
-    kPyrDown<<<grid, 256, 0, stream>>>(srcA, sstepA, srcB, sstepB, sw, sh, dstA, dstepA, dstB,
-                                       dstepB, dw, dh);
-    return hipGetLastError();
+    return launchPyrDownBatch(srcA, sstepA, srcB, sstepB, 0, sw, sh, dstA, dstepA, dstB, dstepB,
+                              0, dw, dh, 1, stream);
 }
 
 hipError_t launchPyrDown(const uint8_t* src, int sstep, int sw, int sh, uint8_t* dst, int dstep,

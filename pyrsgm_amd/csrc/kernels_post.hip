@@ -29,6 +29,7 @@ struct PostArgs {
     float* out32;
     float* outc;
     size_t s16, s32, sc;  // bytes
+    Batch batch;
 };
 
 __device__ __forceinline__ int disp2At(const PostArgs& a, int y, int x2, int invalid)
@@ -81,8 +82,17 @@ __device__ __forceinline__ int median9(int p[9])
     return p[4];
 }
 
-__global__ __launch_bounds__(256) void kPost(PostArgs a)
+__global__ __launch_bounds__(256) void kPost(PostArgs args)
 {
+    // This pair's buffers (pair = blockIdx.z).
+    PostArgs a = args;
+    const size_t pr = blockIdx.z;
+    a.disp = pairAt(a.disp, pr, a.batch.ws);
+    a.conf = pairAt(a.conf, pr, a.batch.ws);
+    a.keys = pairAt(a.keys, pr, a.batch.ws);
+    a.out16 = pairAt(a.out16, pr, a.batch.out16);
+    a.out32 = pairAt(a.out32, pr, a.batch.out32);
+    a.outc = pairAt(a.outc, pr, a.batch.outc);
     __shared__ int16_t s_d[kTile + 2][kTile + 2];
     const int tx = threadIdx.x & (kTile - 1), ty = threadIdx.x / kTile;
     const int bx = blockIdx.x * kTile, by = blockIdx.y * kTile;
@@ -166,7 +176,8 @@ hipError_t launchPost(const LevelCall& c, Workspace& ws, hipStream_t s)
     a.s32 = c.s32;
     a.sc = c.sc;
     if (!a.out16 && !a.outc) return hipSuccess;
-    dim3 grid((c.W + kTile - 1) / kTile, (c.H + kTile - 1) / kTile);
+    a.batch = c.batch;
+    dim3 grid((c.W + kTile - 1) / kTile, (c.H + kTile - 1) / kTile, c.batch.n);
     kPost<<<grid, 256, 0, s>>>(a);
     return hipGetLastError();
 }

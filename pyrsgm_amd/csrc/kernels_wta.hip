@@ -42,6 +42,15 @@ __global__ __launch_bounds__(256) void kWTA(PathArgs a)
     constexpr bool PYR = MODE != kSingle;
     const int W = a.W, D = a.D;
     const size_t P = (size_t)W * a.H;
+    const size_t pr = blockIdx.y;  // pair
+    const size_t wsS = a.batch.ws;
+    const int16_t* const Arl = pairAt(a.Arl, pr, wsS);
+    const int16_t* const Alr = pairAt(a.Alr, pr, wsS);
+    const int16_t* const Atb = pairAt(a.Atb, pr, wsS);
+    const int32_t* const labels = pairAt(a.labels, pr, wsS);
+    int16_t* const dispV = pairAt(a.disp, pr, wsS);
+    float* const confV = pairAt(a.conf, pr, wsS);
+    uint32_t* const keys = pairAt(a.keys, pr, wsS);
     const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
     const size_t p = t / Q;
     const int sub = (int)(t - p * Q);
@@ -53,9 +62,9 @@ __global__ __launch_bounds__(256) void kWTA(PathArgs a)
     int my[16];
     {
         int rl[16], lr[16], tb[16];
-        load16(rl, a.Arl + off);
-        if (np >= 2) load16(lr, a.Alr + off);
-        if (np >= 3) load16(tb, a.Atb + off);
+        load16(rl, Arl + off);
+        if (np >= 2) load16(lr, Alr + off);
+        if (np >= 3) load16(tb, Atb + off);
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
             if constexpr (!PYR) {
@@ -118,13 +127,13 @@ __global__ __launch_bounds__(256) void kWTA(PathArgs a)
             cf = fminf(fmaxf(((float)alt - (float)minCost) / denom, 0.0f), 1.0f);
             const int x2 = x - bestD - a.minD;
             if (x2 >= 0 && x2 < W)
-                atomicMin(&a.keys[(size_t)y * W + x2],
+                atomicMin(&keys[(size_t)y * W + x2],
                           ((uint32_t)(minCost + 32768) << 16) | (uint32_t)(0xFFFF - x));
         }
     } else {
         int base = a.minD, valid = 1;
         if constexpr (MODE == kRefine) {
-            const int lw = a.labels[p];
+            const int lw = labels[p];
             base = (int)(int16_t)(lw & 0xFFFF);
             valid = (lw >> 16) & 1;
         }
@@ -152,12 +161,12 @@ __global__ __launch_bounds__(256) void kWTA(PathArgs a)
             cf = c;
             const int x2 = x - realBest;
             if (x2 >= 0 && x2 < W)
-                atomicMin(&a.keys[(size_t)y * W + x2],
+                atomicMin(&keys[(size_t)y * W + x2],
                           ((uint32_t)(minCost + 32768) << 16) | (uint32_t)(0xFFFF - x));
         }
     }
-    a.disp[p] = out;
-    a.conf[p] = cf;
+    dispV[p] = out;
+    confV[p] = cf;
 }
 
 template <int MODE>
@@ -167,13 +176,13 @@ hipError_t dispatchWTA(const PathArgs& a, hipStream_t s)
     const size_t threads = (size_t)a.W * a.H * Q;
     const unsigned blocks = (unsigned)((threads + 255) / 256);
     switch (Q) {
-    case 1: kWTA<1, MODE><<<blocks, 256, 0, s>>>(a); break;
-    case 2: kWTA<2, MODE><<<blocks, 256, 0, s>>>(a); break;
-    case 4: kWTA<4, MODE><<<blocks, 256, 0, s>>>(a); break;
-    case 8: kWTA<8, MODE><<<blocks, 256, 0, s>>>(a); break;
-    case 16: kWTA<16, MODE><<<blocks, 256, 0, s>>>(a); break;
-    case 32: kWTA<32, MODE><<<blocks, 256, 0, s>>>(a); break;
-    case 64: kWTA<64, MODE><<<blocks, 256, 0, s>>>(a); break;
+    case 1: kWTA<1, MODE><<<dim3(blocks, a.batch.n), 256, 0, s>>>(a); break;
+    case 2: kWTA<2, MODE><<<dim3(blocks, a.batch.n), 256, 0, s>>>(a); break;
+    case 4: kWTA<4, MODE><<<dim3(blocks, a.batch.n), 256, 0, s>>>(a); break;
+    case 8: kWTA<8, MODE><<<dim3(blocks, a.batch.n), 256, 0, s>>>(a); break;
+    case 16: kWTA<16, MODE><<<dim3(blocks, a.batch.n), 256, 0, s>>>(a); break;
+    case 32: kWTA<32, MODE><<<dim3(blocks, a.batch.n), 256, 0, s>>>(a); break;
+    case 64: kWTA<64, MODE><<<dim3(blocks, a.batch.n), 256, 0, s>>>(a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -1,4 +1,5 @@
-// matcher_capi.cpp -- plain-C handles on pyrsgm::StereoMatcher (pyrsgm_py.h).
+// matcher_capi.cpp -- plain-C handles on pyrsgm::StereoMatcher (pyrsgm_py.h)
+// and pyrsgm::BatchMatcher (pyrsgm_batch.h).
 #include <stdio.h>
 #include <string.h>
 
@@ -6,6 +7,7 @@
 #include <new>
 
 #include "pyrsgm/pyrsgm.h"
+#include "pyrsgm/pyrsgm_batch.h"
 #include "pyrsgm/pyrsgm_py.h"
 
 namespace {
@@ -156,5 +158,85 @@ size_t pyrsgm_matcher_device_bytes(void* matcher)
 }
 
 const char* pyrsgm_version(void) { return pyrsgm::version(); }
+
+
+// ---- BatchMatcher handles (pyrsgm_batch.h) --------------------------------
+void* pyrsgm_batch_create(int width, int height, int batch, const pyrsgm_params_c* params,
+                          char* err, int errlen)
+{
+    if (!params) {
+        copyErr("pyrsgm: null params", err, errlen);
+        return nullptr;
+    }
+    try {
+        return new pyrsgm::BatchMatcher(width, height, batch, toParams(*params));
+    } catch (const std::exception& e) {
+        copyErr(e.what(), err, errlen);
+    } catch (...) {
+        copyErr("pyrsgm: unknown error", err, errlen);
+    }
+    return nullptr;
+}
+
+void pyrsgm_batch_destroy(void* matcher) { delete static_cast<pyrsgm::BatchMatcher*>(matcher); }
+
+int pyrsgm_batch_compute(void* matcher,
+                         const unsigned char* d_left, size_t left_pitch, size_t left_stride,
+                         const unsigned char* d_right, size_t right_pitch, size_t right_stride,
+                         short* d_disp16, size_t disp16_pitch, size_t disp16_stride,
+                         float* d_disp32, size_t disp32_pitch, size_t disp32_stride,
+                         float* d_conf, size_t conf_pitch, size_t conf_stride,
+                         hipStream_t stream, char* err, int errlen)
+{
+    if (!matcher) return -1;
+    try {
+        static_cast<pyrsgm::BatchMatcher*>(matcher)->compute(
+            {d_left, left_pitch, left_stride}, {d_right, right_pitch, right_stride},
+            {d_disp16, disp16_pitch, disp16_stride}, {d_disp32, disp32_pitch, disp32_stride},
+            {d_conf, conf_pitch, conf_stride}, stream);
+        return 0;
+    } catch (const std::exception& e) {
+        copyErr(e.what(), err, errlen);
+    }
+    return -1;
+}
+
+int pyrsgm_batch_capture(void* matcher,
+                         const unsigned char* d_left, size_t left_pitch, size_t left_stride,
+                         const unsigned char* d_right, size_t right_pitch, size_t right_stride,
+                         short* d_disp16, size_t disp16_pitch, size_t disp16_stride,
+                         float* d_disp32, size_t disp32_pitch, size_t disp32_stride,
+                         float* d_conf, size_t conf_pitch, size_t conf_stride,
+                         char* err, int errlen)
+{
+    if (!matcher) return -1;
+    try {
+        static_cast<pyrsgm::BatchMatcher*>(matcher)->capture(
+            {d_left, left_pitch, left_stride}, {d_right, right_pitch, right_stride},
+            {d_disp16, disp16_pitch, disp16_stride}, {d_disp32, disp32_pitch, disp32_stride},
+            {d_conf, conf_pitch, conf_stride});
+        return 0;
+    } catch (const std::exception& e) {
+        copyErr(e.what(), err, errlen);
+    }
+    return -1;
+}
+
+int pyrsgm_batch_launch(void* matcher, hipStream_t stream, char* err, int errlen)
+{
+    if (!matcher) return -1;
+    try {
+        static_cast<pyrsgm::BatchMatcher*>(matcher)->launch(stream);
+        return 0;
+    } catch (const std::exception& e) {
+        copyErr(e.what(), err, errlen);
+    }
+    return -1;
+}
+
+size_t pyrsgm_batch_device_bytes(void* matcher)
+{
+    return matcher ? static_cast<pyrsgm::BatchMatcher*>(matcher)->deviceMemoryBytes() : 0;
+}
 
 } // extern "C"

@@ -348,8 +348,8 @@ __device__ __forceinline__ void waveLdsFence()
 
 // MODE: kSingle, kCoarse or kRefine.
 template <int G, int K, int MODE, bool ADAPT, bool EXACT, int DIR>
-__device__ __forceinline__ void runLine(const PathArgs& a, int lineGroup, const int* s_lut,
-                                        int* s_xchg, char* s_lds)
+__device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int lineGroup,
+                                        const int* s_lut, int* s_xchg, char* s_lds)
 {
     constexpr int LPW = 64 / G;
     constexpr int GK = G * K;
@@ -388,10 +388,12 @@ __device__ __forceinline__ void runLine(const PathArgs& a, int lineGroup, const 
     constexpr uint32_t wElem = REFINE ? 4 : 1;
     auto lineStride = [&](uint32_t rowPitch, uint32_t elem) { return VERT ? elem : rowPitch; };
     auto stepStride = [&](uint32_t rowPitch, uint32_t elem) { return VERT ? rowPitch : elem; };
-    const uint32_t vVol = lineOk ? (uint32_t)j * lineStride(volRow, VB) + (uint32_t)g * K * 2 : kOffLimit;
-    const uint32_t wLine = (uint32_t)j * lineStride(wRow, wElem);
     const uint32_t wStep = stepStride(wRow, wElem);
-    int16_t* const out = DIR == kLR ? a.Alr : (DIR == kRL ? a.Arl : a.Atb);
+    // This pair's streams.
+    const int16_t* const costV = pairAt(a.C, pair, a.batch.ws);
+    const int32_t* const labelsV = pairAt(a.labels, pair, a.batch.ws);
+    const uint8_t* const imgV = pairAt(a.img, pair, a.batch.left);
+    int16_t* const out = pairAt(DIR == kLR ? a.Alr : (DIR == kRL ? a.Arl : a.Atb), pair, a.batch.ws);
 
     // full: the chunk [sc, sc+U) lies inside the line, so every offset is a
     // compile-time multiple of the step stride (an instruction immediate for
@@ -437,7 +439,7 @@ __device__ __forceinline__ void runLine(const PathArgs& a, int lineGroup, const 
         const int nRows = VERT ? nSteps : nValid;
         const int nCols = VERT ? nValid : nSteps;
         const ChunkAddr<U> cv = chunk(sc, volRow, VB, VB, full);
-        const auto rC = rsrc(reinterpret_cast<const char*>(a.C) + cv.origin, cv.nr);
+        const auto rC = rsrc(reinterpret_cast<const char*>(costV) + cv.origin, cv.nr);
 #pragma unroll
         for (int i = 0; i < TL::NQ; ++i) {
             const int pc = lane + 64 * i;
@@ -455,7 +457,7 @@ __device__ __forceinline__ void runLine(const PathArgs& a, int lineGroup, const 
             const int rel = RL ? cw.sLast - sc - r : r;
             const bool ok = jm < nValid && r < nSteps;
             const uint32_t voff = ok ? (uint32_t)jm * lineStride(wRow, wElem) + (uint32_t)rel * wStep : kOffLimit;
-            const char* base = REFINE ? reinterpret_cast<const char*>(a.labels) : reinterpret_cast<const char*>(a.img);
+            const char* base = REFINE ? reinterpret_cast<const char*>(labelsV) : reinterpret_cast<const char*>(imgV);
             const auto rw = rsrc(base + cw.origin, cw.nr);
             m.w = REFINE ? (int)__builtin_amdgcn_raw_buffer_load_b32(rw, voff, 0, 0)
                          : (int)__builtin_amdgcn_raw_buffer_load_b8(rw, voff, 0, 0);
@@ -793,12 +795,13 @@ __global__ __launch_bounds__(128) void kPaths(PathArgs a)
         __syncthreads();
     }
     const int b = blockIdx.x;
+    const size_t pr = blockIdx.y;  // pair
     if (b < a.nLRBlocks)
-        runLine<G, K, MODE, ADAPT, EXACT, kLR>(a, b, s_lut, s_xchg, s_lds);
+        runLine<G, K, MODE, ADAPT, EXACT, kLR>(a, pr,  b, s_lut, s_xchg, s_lds);
     else if (b < a.nLRBlocks + a.nRLBlocks)
-        runLine<G, K, MODE, ADAPT, EXACT, kRL>(a, b - a.nLRBlocks, s_lut, s_xchg, s_lds);
+        runLine<G, K, MODE, ADAPT, EXACT, kRL>(a, pr,  b - a.nLRBlocks, s_lut, s_xchg, s_lds);
     else
-        runLine<G, K, MODE, ADAPT, EXACT, kTB>(a, b - a.nLRBlocks - a.nRLBlocks, s_lut, s_xchg, s_lds);
+        runLine<G, K, MODE, ADAPT, EXACT, kTB>(a, pr,  b - a.nLRBlocks - a.nRLBlocks, s_lut, s_xchg, s_lds);
 }
 
 inline int linesPerWave(int Dp) { return Dp <= 16 ? 4 : (Dp <= 32 ? 2 : 1); }
@@ -813,13 +816,13 @@ hipError_t dispatchPaths(const PathArgs& a, hipStream_t s)
     const int nTB = a.numPaths >= 3 ? (a.W + lpw - 1) / lpw : 0;
     const int blocks = b.nLRBlocks + b.nRLBlocks + nTB;
     switch (a.Dp) {
-    case 16: kPaths<16, 1, MODE, ADAPT, EXACT><<<blocks, 128, 0, s>>>(b); break;
-    case 32: kPaths<32, 1, MODE, ADAPT, EXACT><<<blocks, 128, 0, s>>>(b); break;
-    case 64: kPaths<64, 1, MODE, ADAPT, EXACT><<<blocks, 128, 0, s>>>(b); break;
-    case 128: kPaths<64, 2, MODE, ADAPT, EXACT><<<blocks, 128, 0, s>>>(b); break;
-    case 256: kPaths<64, 4, MODE, ADAPT, EXACT><<<blocks, 128, 0, s>>>(b); break;
-    case 512: kPaths<64, 8, MODE, ADAPT, EXACT><<<blocks, 128, 0, s>>>(b); break;
-    case 1024: kPaths<64, 16, MODE, ADAPT, EXACT><<<blocks, 128, 0, s>>>(b); break;
+    case 16: kPaths<16, 1, MODE, ADAPT, EXACT><<<dim3(blocks, a.batch.n), 128, 0, s>>>(b); break;
+    case 32: kPaths<32, 1, MODE, ADAPT, EXACT><<<dim3(blocks, a.batch.n), 128, 0, s>>>(b); break;
+    case 64: kPaths<64, 1, MODE, ADAPT, EXACT><<<dim3(blocks, a.batch.n), 128, 0, s>>>(b); break;
+    case 128: kPaths<64, 2, MODE, ADAPT, EXACT><<<dim3(blocks, a.batch.n), 128, 0, s>>>(b); break;
+    case 256: kPaths<64, 4, MODE, ADAPT, EXACT><<<dim3(blocks, a.batch.n), 128, 0, s>>>(b); break;
+    case 512: kPaths<64, 8, MODE, ADAPT, EXACT><<<dim3(blocks, a.batch.n), 128, 0, s>>>(b); break;
+    case 1024: kPaths<64, 16, MODE, ADAPT, EXACT><<<dim3(blocks, a.batch.n), 128, 0, s>>>(b); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -1,4 +1,4 @@
-// stereo_matcher.cpp -- pyrsgm::StereoMatcher on top of the C-ABI.
+// stereo_matcher.cpp -- pyrsgm::StereoMatcher and pyrsgm::BatchMatcher.
 //
 // Same orchestration contract as the reference (stereo_matcher.cpp): level
 // geometry and parameter validation at construction, every device buffer
@@ -6,6 +6,7 @@
 // on the caller's stream.  The pyrDown chain processes both views in one
 // launch per level.
 #include "pyrsgm/pyrsgm.h"
+#include "pyrsgm/pyrsgm_batch.h"
 
 #include <algorithm>
 #include <array>
@@ -108,8 +109,11 @@ Params validateParams(const Params& in)
     return p;
 }
 
-struct StereoMatcher::Impl {
-    int width = 0, height = 0;
+// Level geometry, device state and the coarse-to-fine sequence for n pairs of
+// one size; StereoMatcher is n = 1, BatchMatcher any n.  Per-pair images of
+// one level are stacked: pair i's rows follow pair i-1's.
+struct MatcherCore {
+    int width = 0, height = 0, n = 1;
     Params params;
     int levels = 0;
     std::array<void*, kMaxPyramidLevels + 1> workspaces{};
@@ -119,11 +123,12 @@ struct StereoMatcher::Impl {
     std::array<LevelInfo, kMaxPyramidLevels + 1> info{};
     size_t workspaceBytes = 0;
 
-    Impl(int w, int h, const Params& p) : width(w), height(h), params(validateParams(p)),
-                                          levels(params.pyramid_levels)
+    MatcherCore(int w, int h, int pairs, const Params& p)
+        : width(w), height(h), n(pairs), params(validateParams(p)), levels(params.pyramid_levels)
     {
         if (w <= 0 || h <= 0) throw std::invalid_argument("pyrsgm: empty image size");
         if (w > pyr::kMaxWidth) throw std::invalid_argument("pyrsgm: width must be <= 65535");
+        if (pairs < 1) throw std::invalid_argument("pyrsgm: batch must be >= 1");
         // Reference: stereo_matcher.cpp:169-193.
         int lw = w, lh = h;
         for (int level = 0; level <= levels; ++level) {
@@ -153,24 +158,25 @@ struct StereoMatcher::Impl {
                 workspaces[level] = pyrsgm_create_workspace();
                 if (!workspaces[level]) throw std::runtime_error("pyrsgm: workspace allocation failed");
                 const LevelInfo& li = info[level];
-                if (pyrsgm_workspace_reserve(workspaces[level], li.width, li.height, li.num_labels) != 0)
+                if (pyr::ensureWorkspace(static_cast<pyr::Workspace*>(workspaces[level]), li.width,
+                                         li.height, li.num_labels, n) != 0)
                     throwHip("workspace reserve", hipErrorOutOfMemory);
-                workspaceBytes += pyr::workspaceBytes(li.width, li.height, li.num_labels);
+                workspaceBytes += pyr::workspaceBytes(li.width, li.height, li.num_labels) * n;
                 if (level > 0) {
-                    pyrLeft[level].create(li.width, li.height);
-                    pyrRight[level].create(li.width, li.height);
-                    disp16[level].create(li.width, li.height);
-                    conf[level].create(li.width, li.height);
+                    pyrLeft[level].create(li.width, li.height * n);
+                    pyrRight[level].create(li.width, li.height * n);
+                    disp16[level].create(li.width, li.height * n);
+                    conf[level].create(li.width, li.height * n);
                 }
             }
-            conf[0].create(w, h);
+            conf[0].create(w, h * n);
         } catch (...) {
             destroy();
             throw;
         }
     }
 
-    ~Impl() { destroy(); }
+    ~MatcherCore() { destroy(); }
 
     void destroy()
     {
@@ -187,74 +193,110 @@ struct StereoMatcher::Impl {
         return (params.lr_check ? PYRSGM_POST_LRCHECK : 0) | (params.median_filter ? PYRSGM_POST_MEDIAN : 0);
     }
 
-    // Reference: stereo_matcher.cpp:238-338.
-    void compute(const uint8_t* d_left, size_t left_pitch, const uint8_t* d_right,
-                 size_t right_pitch, int16_t* d_disp16, size_t disp16_pitch, float* d_disp32,
-                 size_t disp32_pitch, float* d_conf, size_t conf_pitch, hipStream_t stream)
+    // Pair stride of a stacked per-level image.
+    template <class T>
+    size_t stride(const DeviceImage<T>& im, int level) const
     {
-        if (!d_left || !d_right) throw std::invalid_argument("pyrsgm: null input image");
-        if (!d_disp16 && !d_disp32 && !d_conf) return;
-        float* outConf = d_conf;
-        size_t outConfPitch = conf_pitch;
-        if (!outConf && d_disp32) {
-            outConf = conf[0].data;
-            outConfPitch = conf[0].pitch;
-        }
+        return im.pitch * (size_t)info[level].height;
+    }
+
+    // Reference: stereo_matcher.cpp:238-338.
+    void compute(BatchBuffer<const uint8_t> L, BatchBuffer<const uint8_t> R,
+                 BatchBuffer<int16_t> d16, BatchBuffer<float> d32, BatchBuffer<float> cf,
+                 hipStream_t stream)
+    {
+        if (!L.data || !R.data) throw std::invalid_argument("pyrsgm: null input image");
+        if (!d16.data && !d32.data && !cf.data) return;
+        BatchBuffer<float> oc = cf;
+        if (!oc.data && d32.data) oc = {conf[0].data, conf[0].pitch, stride(conf[0], 0)};
         const int costMode = params.cost_mode == CostMode::Census ? PYRSGM_COST_CENSUS : PYRSGM_COST_BT;
         if (levels == 0) {
-            const int ret = pyrsgm_compute_single_scale(
-                workspaces[0], d_left, d_right, d_disp16, d_disp32, outConf, left_pitch,
-                right_pitch, disp16_pitch, disp32_pitch, outConfPitch, width, height,
-                params.num_disparities, params.min_disparity, params.block_size, params.p1,
-                params.p2, params.uniqueness_ratio, params.disp12_max_diff, params.pre_filter_cap,
-                costMode, postFlags(0), params.num_paths, params.adaptive_p2 ? 1 : 0,
-                params.p2_alpha, params.p2_beta, params.p2_lower_mult, params.p2_upper_mult, stream);
+            pyr::Batch b;
+            b.n = n;
+            b.left = L.pair_stride;
+            b.right = R.pair_stride;
+            b.out16 = d16.pair_stride;
+            b.out32 = d32.pair_stride;
+            b.outc = oc.pair_stride;
+            const int ret = pyr::computeSingleScale(
+                workspaces[0], L.data, R.data, d16.data, d32.data, oc.data, L.pitch, R.pitch,
+                d16.pitch, d32.pitch, oc.pitch, width, height, params.num_disparities,
+                params.min_disparity, params.block_size, params.p1, params.p2,
+                params.uniqueness_ratio, params.disp12_max_diff, params.pre_filter_cap, costMode,
+                postFlags(0), params.num_paths, params.adaptive_p2 ? 1 : 0, params.p2_alpha,
+                params.p2_beta, params.p2_lower_mult, params.p2_upper_mult, b, stream);
             if (ret != 0) throwHip("single-scale SGM", hipGetLastError());
             return;
         }
-        const uint8_t* pl = d_left;
-        size_t plp = left_pitch;
-        const uint8_t* pr = d_right;
-        size_t prp = right_pitch;
+        // pyrDown chain: both views in one launch when their pair strides agree.
         for (int level = 1; level <= levels; ++level) {
             const LevelInfo& parent = info[level - 1];
             const LevelInfo& li = info[level];
+            const bool top = level == 1;
+            const uint8_t* sl = top ? L.data : pyrLeft[level - 1].data;
+            const uint8_t* sr = top ? R.data : pyrRight[level - 1].data;
+            const size_t slp = top ? L.pitch : pyrLeft[level - 1].pitch;
+            const size_t srp = top ? R.pitch : pyrRight[level - 1].pitch;
+            const size_t sls = top ? L.pair_stride : stride(pyrLeft[level - 1], level - 1);
+            const size_t srs = top ? R.pair_stride : stride(pyrRight[level - 1], level - 1);
+            const size_t ds = stride(pyrLeft[level], level);
             pyr::profBegin(PYRSGM_STAGE_PYRDOWN, li.width, li.height, 0, stream);
-            const hipError_t e = pyr::launchPyrDown2(
-                pl, (int)plp, pr, (int)prp, parent.width, parent.height, pyrLeft[level].data,
-                (int)pyrLeft[level].pitch, pyrRight[level].data, (int)pyrRight[level].pitch,
-                li.width, li.height, stream);
+            hipError_t e;
+            if (sls == srs) {
+                e = pyr::launchPyrDownBatch(sl, (int)slp, sr, (int)srp, sls, parent.width,
+                                            parent.height, pyrLeft[level].data,
+                                            (int)pyrLeft[level].pitch, pyrRight[level].data,
+                                            (int)pyrRight[level].pitch, ds, li.width, li.height,
+                                            n, stream);
+            } else {
+                e = pyr::launchPyrDownBatch(sl, (int)slp, nullptr, 0, sls, parent.width,
+                                            parent.height, pyrLeft[level].data,
+                                            (int)pyrLeft[level].pitch, nullptr, 0, ds, li.width,
+                                            li.height, n, stream);
+                if (e == hipSuccess)
+                    e = pyr::launchPyrDownBatch(sr, (int)srp, nullptr, 0, srs, parent.width,
+                                                parent.height, pyrRight[level].data,
+                                                (int)pyrRight[level].pitch, nullptr, 0, ds,
+                                                li.width, li.height, n, stream);
+            }
             pyr::profEnd(stream);
             if (e != hipSuccess) throwHip("pyrDown", e);
-            pl = pyrLeft[level].data;
-            plp = pyrLeft[level].pitch;
-            pr = pyrRight[level].data;
-            prp = pyrRight[level].pitch;
         }
         for (int level = levels; level >= 0; --level) {
             const LevelInfo& li = info[level];
             const bool refine = li.refine;
-            const uint8_t* L = level == 0 ? d_left : pyrLeft[level].data;
-            const uint8_t* R = level == 0 ? d_right : pyrRight[level].data;
-            const size_t lp = level == 0 ? left_pitch : pyrLeft[level].pitch;
-            const size_t rp = level == 0 ? right_pitch : pyrRight[level].pitch;
+            const bool fine = level == 0;
+            const uint8_t* Lp = fine ? L.data : pyrLeft[level].data;
+            const uint8_t* Rp = fine ? R.data : pyrRight[level].data;
+            const size_t lp = fine ? L.pitch : pyrLeft[level].pitch;
+            const size_t rp = fine ? R.pitch : pyrRight[level].pitch;
             const DeviceImage<int16_t>* pd = refine ? &disp16[level + 1] : nullptr;
             const DeviceImage<float>* pc = refine ? &conf[level + 1] : nullptr;
-            int16_t* o16 = level == 0 ? d_disp16 : disp16[level].data;
-            const size_t o16p = level == 0 ? disp16_pitch : disp16[level].pitch;
-            float* o32 = level == 0 ? d_disp32 : nullptr;
-            const size_t o32p = level == 0 ? disp32_pitch : 0;
-            float* oc = level == 0 ? outConf : conf[level].data;
-            const size_t ocp = level == 0 ? outConfPitch : conf[level].pitch;
-            const int ret = pyrsgm_compute_level(
-                workspaces[level], L, R, pd ? pd->data : nullptr, pc ? pc->data : nullptr, o16, o32,
-                oc, lp, rp, pd ? pd->pitch : 0, pc ? pc->pitch : 0, o16p, o32p, ocp, li.width,
-                li.height, pd ? pd->width : 0, pd ? pd->height : 0, li.num_labels,
-                params.min_disparity, li.max_disparity, li.block_size, params.p1, params.p2,
-                params.uniqueness_ratio, params.disp12_max_diff, params.pre_filter_cap, costMode,
-                postFlags(level), params.num_paths, refine ? 1 : 0, params.pyramid_min_confidence,
+            int16_t* o16 = fine ? d16.data : disp16[level].data;
+            const size_t o16p = fine ? d16.pitch : disp16[level].pitch;
+            float* o32 = fine ? d32.data : nullptr;
+            const size_t o32p = fine ? d32.pitch : 0;
+            float* oc2 = fine ? oc.data : conf[level].data;
+            const size_t ocp = fine ? oc.pitch : conf[level].pitch;
+            pyr::Batch b;
+            b.n = n;
+            b.left = fine ? L.pair_stride : stride(pyrLeft[level], level);
+            b.right = fine ? R.pair_stride : stride(pyrRight[level], level);
+            b.pdisp = refine ? stride(*pd, level + 1) : 0;
+            b.pconf = refine ? stride(*pc, level + 1) : 0;
+            b.out16 = fine ? d16.pair_stride : stride(disp16[level], level);
+            b.out32 = fine ? d32.pair_stride : 0;
+            b.outc = fine ? oc.pair_stride : stride(conf[level], level);
+            const int ret = pyr::computeLevel(
+                workspaces[level], Lp, Rp, pd ? pd->data : nullptr, pc ? pc->data : nullptr, o16,
+                o32, oc2, lp, rp, pd ? pd->pitch : 0, pc ? pc->pitch : 0, o16p, o32p, ocp,
+                li.width, li.height, pd ? info[level + 1].width : 0,
+                pd ? info[level + 1].height : 0, li.num_labels, params.min_disparity,
+                li.max_disparity, li.block_size, params.p1, params.p2, params.uniqueness_ratio,
+                params.disp12_max_diff, params.pre_filter_cap, costMode, postFlags(level),
+                params.num_paths, refine ? 1 : 0, params.pyramid_min_confidence,
                 params.adaptive_p2 ? 1 : 0, params.p2_alpha, params.p2_beta, params.p2_lower_mult,
-                params.p2_upper_mult, stream);
+                params.p2_upper_mult, b, stream);
             if (ret != 0) {
                 std::ostringstream oss;
                 oss << "pyramid level " << level << " (" << li.width << "x" << li.height
@@ -271,6 +313,19 @@ struct StereoMatcher::Impl {
             total += pyrLeft[level].bytes() + pyrRight[level].bytes() + disp16[level].bytes() +
                      conf[level].bytes();
         return total;
+    }
+};
+
+struct StereoMatcher::Impl : MatcherCore {
+    Impl(int w, int h, const Params& p) : MatcherCore(w, h, 1, p) {}
+
+    void compute(const uint8_t* d_left, size_t left_pitch, const uint8_t* d_right,
+                 size_t right_pitch, int16_t* d_disp16, size_t disp16_pitch, float* d_disp32,
+                 size_t disp32_pitch, float* d_conf, size_t conf_pitch, hipStream_t stream)
+    {
+        MatcherCore::compute({d_left, left_pitch, 0}, {d_right, right_pitch, 0},
+                             {d_disp16, disp16_pitch, 0}, {d_disp32, disp32_pitch, 0},
+                             {d_conf, conf_pitch, 0}, stream);
     }
 };
 
@@ -299,6 +354,84 @@ LevelInfo StereoMatcher::levelInfo(int level) const
     return impl_->info[level];
 }
 size_t StereoMatcher::deviceMemoryBytes() const { return impl_->deviceBytes(); }
+
+// ---- BatchMatcher ---------------------------------------------------------
+struct BatchMatcher::Impl : MatcherCore {
+    hipStream_t captureStream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+
+    Impl(int w, int h, int n, const Params& p) : MatcherCore(w, h, n, p) {}
+    ~Impl() { dropGraph(); if (captureStream) (void)hipStreamDestroy(captureStream); }
+
+    void dropGraph()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+
+    // The sequence is recorded on a stream of the matcher's own (capture needs
+    // a stream other than the legacy default one); launch() replays it on any.
+    void capture(BatchBuffer<const uint8_t> L, BatchBuffer<const uint8_t> R,
+                 BatchBuffer<int16_t> d16, BatchBuffer<float> d32, BatchBuffer<float> cf)
+    {
+        dropGraph();
+        if (!captureStream) check(hipStreamCreateWithFlags(&captureStream, hipStreamNonBlocking),
+                                  "hipStreamCreateWithFlags");
+        check(hipStreamBeginCapture(captureStream, hipStreamCaptureModeThreadLocal),
+              "hipStreamBeginCapture");
+        try {
+            compute(L, R, d16, d32, cf, captureStream);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(captureStream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        check(hipStreamEndCapture(captureStream, &graph), "hipStreamEndCapture");
+        check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    }
+};
+
+BatchMatcher::BatchMatcher(int width, int height, int batch, const Params& params)
+    : impl_(new Impl(width, height, batch, params)) {}
+BatchMatcher::~BatchMatcher() = default;
+BatchMatcher::BatchMatcher(BatchMatcher&&) noexcept = default;
+BatchMatcher& BatchMatcher::operator=(BatchMatcher&&) noexcept = default;
+
+void BatchMatcher::compute(BatchBuffer<const uint8_t> left, BatchBuffer<const uint8_t> right,
+                           BatchBuffer<int16_t> disp16, BatchBuffer<float> disp32,
+                           BatchBuffer<float> conf, hipStream_t stream)
+{
+    impl_->compute(left, right, disp16, disp32, conf, stream);
+}
+
+void BatchMatcher::capture(BatchBuffer<const uint8_t> left, BatchBuffer<const uint8_t> right,
+                           BatchBuffer<int16_t> disp16, BatchBuffer<float> disp32,
+                           BatchBuffer<float> conf)
+{
+    impl_->capture(left, right, disp16, disp32, conf);
+}
+
+void BatchMatcher::launch(hipStream_t stream)
+{
+    if (!impl_->exec) throw std::logic_error("pyrsgm: BatchMatcher::launch before capture");
+    check(hipGraphLaunch(impl_->exec, stream), "hipGraphLaunch");
+}
+
+int BatchMatcher::batch() const { return impl_->n; }
+int BatchMatcher::width() const { return impl_->width; }
+int BatchMatcher::height() const { return impl_->height; }
+const Params& BatchMatcher::params() const { return impl_->params; }
+int BatchMatcher::numLevels() const { return impl_->levels + 1; }
+LevelInfo BatchMatcher::levelInfo(int level) const
+{
+    if (level < 0 || level > impl_->levels) throw std::out_of_range("pyrsgm: level");
+    return impl_->info[level];
+}
+size_t BatchMatcher::deviceMemoryBytes() const { return impl_->deviceBytes(); }
 
 const char* version() { return "0.1.0"; }
 

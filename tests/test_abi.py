@@ -27,7 +27,7 @@ FAKE = C.c_void_p(0x1000)  # non-null, never dereferenced
 def declared_c_functions():
     """Names of the functions declared in the C headers (extern "C" blocks)."""
     names = set()
-    for h in ("pyrsgm_c.h", "pyrsgm_py.h", "pyrsgm_prof.h"):
+    for h in ("pyrsgm_c.h", "pyrsgm_py.h", "pyrsgm_prof.h", "pyrsgm_batch.h"):
         text = open(os.path.join(INC, h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         for m in re.finditer(r"\b(pyrsgm_[a-z0-9_]+)\s*\(", text):
@@ -63,8 +63,23 @@ def test_library_exports_the_cpp_api():
                 "pyrsgm::StereoMatcher::levelInfo(int) const",
                 "pyrsgm::StereoMatcher::numLevels() const",
                 "pyrsgm::StereoMatcher::deviceMemoryBytes() const",
+                "pyrsgm::BatchMatcher::BatchMatcher(int, int, int, pyrsgm::Params const&)",
+                "pyrsgm::BatchMatcher::compute(",
+                "pyrsgm::BatchMatcher::capture(",
+                "pyrsgm::BatchMatcher::launch(",
+                "pyrsgm::BatchMatcher::batch() const",
                 "pyrsgm::version()"):
         assert sym in out, sym
+
+
+def test_batch_rejects_bad_sizes_on_cpu():
+    # Validation happens before any device allocation.
+    with pytest.raises(ValueError):
+        P.BatchMatcher(64, 32, 0, P.Params(num_disparities=64))
+    with pytest.raises(ValueError):
+        P.BatchMatcher(0, 32, 2, P.Params(num_disparities=64))
+    with pytest.raises(ValueError):
+        P.BatchMatcher(64, 32, 2, P.Params(p1=10, p2=5))
 
 
 def test_version():

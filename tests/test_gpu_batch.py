@@ -1,0 +1,132 @@
+"""The batched form: N pairs per launch must equal N single-pair matches, bit for bit.
+
+Every pair gets its own synthetic scene (seeded per pair).  The single-pair
+results come from StereoMatcher (whose own parity with the oracle is tested in
+test_gpu_parity.py); pair 0 is also compared with the oracle directly.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pyrsgm_amd import synth
+from tests import gpu_util as U
+
+pytestmark = pytest.mark.gpu
+
+import torch
+
+import pyrsgm_amd as P
+
+PAD = 5  # extra columns, so every batched tensor has a row pitch != width
+
+
+def batch_tensor(arrays, dtype, fill=0):
+    n, (h, w) = len(arrays), arrays[0].shape if arrays else (0, 0)
+    t = torch.full((n, h, w + PAD), fill, dtype=dtype, device="cuda")
+    for i, a in enumerate(arrays):
+        t[i, :, :w] = torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    return t[:, :, :w]
+
+
+def out_tensor(n, h, w, dtype, fill):
+    return torch.full((n, h, w + PAD), fill, dtype=dtype, device="cuda")[:, :, :w]
+
+
+def pairs(n, w, h, maxd, seed0):
+    ps = [synth.make_pair(w, h, maxd, seed=seed0 + i) for i in range(n)]
+    return [p.left for p in ps], [p.right for p in ps]
+
+
+def singles(lefts, rights, params):
+    return [U.matcher(l, r, params) for l, r in zip(lefts, rights)]
+
+
+def run_batch(bm, L, R, n, h, w):
+    d16 = out_tensor(n, h, w, torch.int16, 1234)
+    d32 = out_tensor(n, h, w, torch.float32, -7.0)
+    cf = out_tensor(n, h, w, torch.float32, -7.0)
+    bm.compute(L, R, d16, d32, cf)
+    torch.cuda.synchronize()
+    return d16, d32, cf
+
+
+CASES = [
+    # (n, w, h, D, levels, cost, adaptive, median)
+    (3, 200, 120, 64, 1, 1, 1, 1),
+    (4, 241, 135, 128, 2, 1, 1, 1),
+    (2, 160, 90, 64, 0, 0, 0, 1),
+    (3, 192, 108, 64, 2, 0, 1, 0),
+]
+
+
+@pytest.mark.parametrize("n,w,h,D,levels,cost,adaptive,median", CASES)
+def test_batch_equals_single_pairs(n, w, h, D, levels, cost, adaptive, median):
+    lefts, rights = pairs(n, w, h, D - 1, seed0=500 + w)
+    params = P.validate_params(P.Params(
+        num_disparities=D, pyramid_levels=levels, cost_mode=P.CostMode(cost),
+        p1=8 if cost else 300, p2=32 if cost else 2400, adaptive_p2=bool(adaptive),
+        median_filter=bool(median)))
+    bm = P.BatchMatcher(w, h, n, params)
+    d16, d32, cf = run_batch(bm, batch_tensor(lefts, torch.uint8), batch_tensor(rights, torch.uint8),
+                             n, h, w)
+    want = singles(lefts, rights, params)
+    for i in range(n):
+        U.assert_same(f"pair {i} disp16", d16[i].cpu().numpy(), want[i][0])
+        U.assert_same(f"pair {i} disp32", d32[i].cpu().numpy(), want[i][1])
+        U.assert_same(f"pair {i} conf", cf[i].cpu().numpy(), want[i][2])
+    ref = O.matcher_compute(lefts[0], rights[0], params)
+    U.assert_same("pair 0 vs oracle", d16[0].cpu().numpy(), ref.disp16)
+    bm.close()
+
+
+def test_batch_views_with_different_strides():
+    """Left and right batches with different pair strides (pyrDown per view)."""
+    n, w, h, D = 3, 180, 100, 64
+    lefts, rights = pairs(n, w, h, D - 1, seed0=77)
+    params = P.validate_params(P.Params(num_disparities=D, pyramid_levels=2,
+                                        cost_mode=P.CostMode.Census, p1=8, p2=32))
+    L = batch_tensor(lefts, torch.uint8)
+    Rbig = torch.zeros((n, h + 3, w + 11), dtype=torch.uint8, device="cuda")
+    for i, r in enumerate(rights):
+        Rbig[i, :h, :w] = torch.from_numpy(r).cuda()
+    R = Rbig[:, :h, :w]
+    bm = P.BatchMatcher(w, h, n, params)
+    d16, d32, cf = run_batch(bm, L, R, n, h, w)
+    want = singles(lefts, rights, params)
+    for i in range(n):
+        U.assert_same(f"pair {i} disp16", d16[i].cpu().numpy(), want[i][0])
+        U.assert_same(f"pair {i} conf", cf[i].cpu().numpy(), want[i][2])
+    bm.close()
+
+
+def test_batch_graph_replays_match_eager():
+    """Whole pyramid captured once; replayed several times on new inputs, with eager
+    work and synchronisation in between; every replay equals the eager result."""
+    n, w, h, D = 3, 200, 112, 128
+    params = P.validate_params(P.Params(num_disparities=D, pyramid_levels=2,
+                                        cost_mode=P.CostMode.Census, p1=8, p2=32,
+                                        adaptive_p2=True))
+    bm = P.BatchMatcher(w, h, n, params)
+    eager = P.BatchMatcher(w, h, n, params)
+    L = out_tensor(n, h, w, torch.uint8, 0)
+    R = out_tensor(n, h, w, torch.uint8, 0)
+    g16 = out_tensor(n, h, w, torch.int16, 1234)
+    g32 = out_tensor(n, h, w, torch.float32, -7.0)
+    gcf = out_tensor(n, h, w, torch.float32, -7.0)
+    bm.capture(L, R, g16, g32, gcf)
+    for rep in range(3):
+        lefts, rights = pairs(n, w, h, D - 1, seed0=900 + 10 * rep)
+        L.copy_(batch_tensor(lefts, torch.uint8))
+        R.copy_(batch_tensor(rights, torch.uint8))
+        torch.cuda.synchronize()
+        bm.launch()
+        torch.cuda.synchronize()
+        e16, e32, ecf = run_batch(eager, L, R, n, h, w)   # eager work between replays
+        U.assert_same(f"replay {rep} disp16", g16.cpu().numpy(), e16.cpu().numpy())
+        U.assert_same(f"replay {rep} disp32", g32.cpu().numpy(), e32.cpu().numpy())
+        U.assert_same(f"replay {rep} conf", gcf.cpu().numpy(), ecf.cpu().numpy())
+        if rep == 0:
+            ref = O.matcher_compute(lefts[1], rights[1], params)
+            U.assert_same("replay 0 pair 1 vs oracle", g16[1].cpu().numpy(), ref.disp16)
+    bm.close()
+    eager.close()

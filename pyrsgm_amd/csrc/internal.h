@@ -45,7 +45,7 @@ struct Workspace {
     size_t slice = 0;             // bytes per slice
     void* base = nullptr;
     size_t bytes = 0;
-    uint2* desc = nullptr;      // [2][P] census words (x) or packed BT intervals (x: grad, y: intensity)
+    uint2* desc = nullptr;      // census: uint32 [2][P] words; BT: uint2 [2][P] intervals (x: grad, y: intensity)
     int16_t* C = nullptr;       // [P][Dp] matching cost
     int16_t* Alr = nullptr;     // [P][Dp] L->R path costs (also the BT row-sum scratch)
     int16_t* Atb = nullptr;     // [P][Dp] T->B path costs

@@ -1,7 +1,7 @@
 // kernels_cost.hip -- per-pixel descriptors, matching-cost volumes and pyrDown.
 //
-// Descriptors (kPrep): one uint2 per pixel per view.  Census: x = 5x5 census
-// word (sgm_kernels.cu:251-298).  BT: the reference's per-channel BT interval
+// Descriptors (kPrepCensus / kPrepBT), per pixel and view.  Census: one 32-bit
+// 5x5 census word (sgm_kernels.cu:251-298).  BT: the reference's per-channel BT interval
 // (u, min, max of the two half-pixel neighbours, sgm_kernels.cu:1002-1028) is
 // a property of one pixel, so it is computed once per pixel and packed as
 // x = gradient channel, y = intensity channel, each (v | v0<<8 | v1<<16).
@@ -55,44 +55,83 @@ __device__ __forceinline__ int btGrad(const uint8_t* img, int step, int W, int H
 }
 
 // ---- descriptors + projection-key reset ----------------------------------
-__global__ __launch_bounds__(256) void kPrep(const uint8_t* left, int lstep, const uint8_t* right,
-                                             int rstep, uint2* desc, uint32_t* keys, int W, int H,
-                                             int census, int cap, Batch b)
+// Census (sgm_kernels.cu:251-298): one 32-bit word per pixel and view, view 0
+// (left) then view 1 (right).  Block: 64 x 16 pixels, four rows per thread; the
+// 68 x 20 window they read (replicate-clamped) is loaded into registers with
+// every load in flight at once, then staged in LDS.
+constexpr int kCenRows = 16;
+
+__global__ __launch_bounds__(256) void kPrepCensus(const uint8_t* left, int lstep,
+                                                   const uint8_t* right, int rstep, uint32_t* cen,
+                                                   uint32_t* keys, int W, int H, Batch b)
+{
+    constexpr int TW = 68, TH = kCenRows + 4, TILE = TW * TH, NL = (TILE + 255) / 256;
+    __shared__ uint8_t s_t[TH][TW];
+    const int z = blockIdx.z & 1;          // view
+    const size_t pr = blockIdx.z >> 1;     // pair
+    const uint8_t* img = z ? pairAt(right, pr, b.right) : pairAt(left, pr, b.left);
+    const int step = z ? rstep : lstep;
+    cen = pairAt(cen, pr, b.ws);
+    keys = pairAt(keys, pr, b.ws);
+    const int x0 = blockIdx.x * 64, y0 = blockIdx.y * kCenRows;
+    uint32_t v[NL];
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+        const int i = threadIdx.x + k * 256;
+        const int ty = i / TW, tx = i - ty * TW;
+        if (i < TILE)
+            v[k] = img[(size_t)clampi(y0 - 2 + ty, 0, H - 1) * step + clampi(x0 - 2 + tx, 0, W - 1)];
+    }
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+        const int i = threadIdx.x + k * 256;
+        const int ty = i / TW, tx = i - ty * TW;
+        if (i < TILE) s_t[ty][tx] = (uint8_t)v[k];
+    }
+    __syncthreads();
+    const int lx = threadIdx.x & 63;
+    const int x = x0 + lx;
+    if (x >= W) return;
+#pragma unroll
+    for (int r = 0; r < kCenRows / 4; ++r) {
+        const int ly = (threadIdx.x >> 6) + 4 * r;
+        const int y = y0 + ly;
+        if (y >= H) break;
+        const int c = s_t[ly + 2][lx + 2];
+        uint32_t cv = 0;
+#pragma unroll
+        for (int dy = 0; dy < 5; ++dy) {
+#pragma unroll
+            for (int dx = 0; dx < 5; ++dx) cv = (cv << 1) | (uint32_t)(s_t[ly + dy][lx + dx] < c);
+        }
+        const size_t p = (size_t)y * W + x;
+        cen[(size_t)z * W * H + p] = cv;
+        if (z == 0) keys[p] = 0xFFFFFFFFu;
+    }
+}
+
+// BT: the packed intervals of a pixel (x: gradient channel, y: intensity).
+__global__ __launch_bounds__(256) void kPrepBT(const uint8_t* left, int lstep, const uint8_t* right,
+                                               int rstep, uint2* desc, uint32_t* keys, int W, int H,
+                                               int cap, Batch b)
 {
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
     const int z = blockIdx.z & 1;          // view
     const size_t pr = blockIdx.z >> 1;     // pair
     if (x >= W || y >= H) return;
-    left = pairAt(left, pr, b.left);
-    right = pairAt(right, pr, b.right);
+    const uint8_t* img = z ? pairAt(right, pr, b.right) : pairAt(left, pr, b.left);
+    const int step = z ? rstep : lstep;
     desc = pairAt(desc, pr, b.ws);
     keys = pairAt(keys, pr, b.ws);
-    const uint8_t* img = z ? right : left;
-    const int step = z ? rstep : lstep;
     const size_t p = (size_t)y * W + x;
-    uint2 out;
-    if (census) {
-        const int c = img[(size_t)y * step + x];
-        uint32_t cv = 0;
-#pragma unroll
-        for (int dy = -2; dy <= 2; ++dy) {
-            const uint8_t* row = img + (size_t)clampi(y + dy, 0, H - 1) * step;
-#pragma unroll
-            for (int dx = -2; dx <= 2; ++dx)
-                cv = (cv << 1) | (uint32_t)(row[clampi(x + dx, 0, W - 1)] < c);
-        }
-        out = make_uint2(cv, 0u);
-    } else {
-        const int ftzero = imax(cap, 15) | 1;
-        const int xm = imax(x - 1, 0), xq = imin(x + 1, W - 1);
-        const int gm = btGrad(img, step, W, H, xm, y, ftzero);
-        const int g0 = btGrad(img, step, W, H, x, y, ftzero);
-        const int gq = btGrad(img, step, W, H, xq, y, ftzero);
-        const uint8_t* row = img + (size_t)y * step;
-        out = make_uint2(btInterval(g0, gm, gq), btInterval(row[x], row[xm], row[xq]));
-    }
-    desc[(size_t)z * W * H + p] = out;
+    const int ftzero = imax(cap, 15) | 1;
+    const int xm = imax(x - 1, 0), xq = imin(x + 1, W - 1);
+    const int gm = btGrad(img, step, W, H, xm, y, ftzero);
+    const int g0 = btGrad(img, step, W, H, x, y, ftzero);
+    const int gq = btGrad(img, step, W, H, xq, y, ftzero);
+    const uint8_t* row = img + (size_t)y * step;
+    desc[(size_t)z * W * H + p] = make_uint2(btInterval(g0, gm, gq), btInterval(row[x], row[xm], row[xq]));
     if (z == 0) keys[p] = 0xFFFFFFFFu;
 }
 
@@ -123,12 +162,12 @@ __device__ __forceinline__ void storeCost8(int16_t* dst, const int (&v)[8])
 // [minD, min(maxD, x)] or whose right column is outside the image is stored as
 // SHRT_MAX: that is exactly the reference's labelValid = geometry && C < MAX
 // (sgm_kernels.cu:1135-1143), so the path kernels test "C < SHRT_MAX" alone.
-__global__ __launch_bounds__(256) void kCensusCost(const uint2* desc, int16_t* C, int W, int H,
+__global__ __launch_bounds__(256) void kCensusCost(const uint32_t* cen, int16_t* C, int W, int H,
                                                    int D, int Dp, int minD, int maxD, int geomMask,
                                                    size_t wsStride)
 {
     extern __shared__ uint32_t s_r[];
-    desc = pairAt(desc, blockIdx.z, wsStride);
+    cen = pairAt(cen, blockIdx.z, wsStride);
     C = pairAt(C, blockIdx.z, wsStride);
     const int tpp = Dp >> 3;
     const int npx = 256 / tpp;
@@ -136,18 +175,18 @@ __global__ __launch_bounds__(256) void kCensusCost(const uint2* desc, int16_t* C
     const int x0 = blockIdx.x * npx;
     const int base2 = x0 - minD - (Dp - 1);
     const int nR = npx + Dp - 1;
-    const uint2* L = desc + (size_t)y * W;
-    const uint2* R = desc + (size_t)W * H + (size_t)y * W;
+    const uint32_t* L = cen + (size_t)y * W;
+    const uint32_t* R = cen + (size_t)W * H + (size_t)y * W;
     for (int i = threadIdx.x; i < nR; i += 256) {
         const int xx = base2 + i;
-        s_r[i] = (xx >= 0 && xx < W) ? R[xx].x : 0u;
+        s_r[i] = (xx >= 0 && xx < W) ? R[xx] : 0u;
     }
     __syncthreads();
     const int px = threadIdx.x / tpp;
     const int d0 = (threadIdx.x - px * tpp) * 8;
     const int x = x0 + px;
     if (x >= W) return;
-    const uint32_t cl = L[x].x;
+    const uint32_t cl = L[x];
     const int hiReal = imin(maxD, x);
     int v[8];
 #pragma unroll
@@ -169,12 +208,10 @@ struct RefineIn {
     float minConf;
 };
 
-__device__ __forceinline__ void refineLabel(const RefineIn& r, int x, int y, int D, int minD,
-                                            int maxD, int& base, int& valid)
+// The window of pixel x from its parent's raw disparity and confidence.
+__device__ __forceinline__ void refineFrom(const RefineIn& r, int raw, float cf, int x, int D,
+                                           int minD, int maxD, int& base, int& valid)
 {
-    const int px = imin(x >> 1, r.PW - 1), py = imin(y >> 1, r.PH - 1);
-    const int raw = r.pdisp[(size_t)py * r.pdpitch + px];
-    const float cf = r.pconf[(size_t)py * r.pcpitch + px];
     const int pinv = (int)(int16_t)((minD - 1) * 16);
     if (raw == pinv || cf < r.minConf) {
         base = minD;
@@ -191,6 +228,22 @@ __device__ __forceinline__ void refineLabel(const RefineIn& r, int x, int y, int
     valid = 1;
 }
 
+__device__ __forceinline__ int parentRaw(const RefineIn& r, int x, int y)
+{
+    return r.pdisp[(size_t)imin(y >> 1, r.PH - 1) * r.pdpitch + imin(x >> 1, r.PW - 1)];
+}
+
+__device__ __forceinline__ float parentConf(const RefineIn& r, int x, int y)
+{
+    return r.pconf[(size_t)imin(y >> 1, r.PH - 1) * r.pcpitch + imin(x >> 1, r.PW - 1)];
+}
+
+__device__ __forceinline__ void refineLabel(const RefineIn& r, int x, int y, int D, int minD,
+                                            int maxD, int& base, int& valid)
+{
+    refineFrom(r, parentRaw(r, x, y), parentConf(r, x, y), x, D, minD, maxD, base, valid);
+}
+
 // Label word of a refinement pixel: bits 0-15 the window base (int16), bit 16
 // the parent-validity flag, bits 24-31 the left-image intensity, which the
 // adaptive-P2 path step reads from here instead of a separate image load.
@@ -200,13 +253,22 @@ __device__ __forceinline__ int32_t labelWord(int base, int valid, uint8_t im)
 }
 
 // ---- Census cost at refinement levels (sgm_kernels.cu:1112-1133) ------------
-__global__ __launch_bounds__(256) void kCensusLocal(const uint2* desc, RefineIn r, int32_t* labels,
+// Block: NPX pixels x (Dp/8) label octets of one row.  A label's right column
+// x - real lies in [x0 - maxD, x0 + NPX - 1 - minD]; when that span is modest
+// (STAGED) the right census words are staged in LDS first, otherwise read from
+// global memory.  Every global load of a thread (staged words, parent
+// disparity and confidence, left census word, intensity) is issued before the
+// first wait.
+template <bool STAGED>
+__global__ __launch_bounds__(256) void kCensusLocal(const uint32_t* cen, RefineIn r, int32_t* labels,
                                                     const uint8_t* img, int istep, int16_t* C,
                                                     int W, int H, int D, int Dp, int minD, int maxD,
                                                     Batch b)
 {
+    constexpr int NPRE = 4;  // staged words per thread loaded ahead
+    extern __shared__ uint32_t s_r[];
     const size_t pr = blockIdx.z;
-    desc = pairAt(desc, pr, b.ws);
+    cen = pairAt(cen, pr, b.ws);
     labels = pairAt(labels, pr, b.ws);
     C = pairAt(C, pr, b.ws);
     img = pairAt(img, pr, b.left);
@@ -215,16 +277,44 @@ __global__ __launch_bounds__(256) void kCensusLocal(const uint2* desc, RefineIn 
     const int tpp = Dp >> 3;
     const int npx = 256 / tpp;
     const int y = blockIdx.y;
+    const int x0 = blockIdx.x * npx;
+    const uint32_t* R = cen + (size_t)W * H + (size_t)y * W;
+    const int xb = x0 - maxD;  // staged column 0
+    const int span = npx + maxD - minD;
+    uint32_t pre[NPRE];
+    if constexpr (STAGED) {
+#pragma unroll
+        for (int k = 0; k < NPRE; ++k) {
+            const int i = threadIdx.x + k * 256;
+            const int xx = xb + i;
+            pre[k] = (i < span && xx >= 0 && xx < W) ? R[xx] : 0u;
+        }
+    }
     const int px = threadIdx.x / tpp;
     const int d0 = (threadIdx.x - px * tpp) * 8;
-    const int x = blockIdx.x * npx + px;
+    const int x = x0 + px;
+    const int xc = imin(x, W - 1);
+    const size_t p = (size_t)y * W + xc;
+    const int raw = parentRaw(r, xc, y);
+    const float cf = parentConf(r, xc, y);
+    const uint32_t cl = cen[p];
+    const uint8_t im = img[(size_t)y * istep + xc];
+    if constexpr (STAGED) {
+#pragma unroll
+        for (int k = 0; k < NPRE; ++k) {
+            const int i = threadIdx.x + k * 256;
+            if (i < span) s_r[i] = pre[k];
+        }
+        for (int i = threadIdx.x + NPRE * 256; i < span; i += 256) {
+            const int xx = xb + i;
+            s_r[i] = (xx >= 0 && xx < W) ? R[xx] : 0u;
+        }
+        __syncthreads();
+    }
     if (x >= W) return;
     int base, valid;
-    refineLabel(r, x, y, D, minD, maxD, base, valid);
-    const size_t p = (size_t)y * W + x;
-    if (d0 == 0) labels[p] = labelWord(base, valid, img[(size_t)y * istep + x]);
-    const uint32_t cl = desc[p].x;
-    const uint2* R = desc + (size_t)W * H + (size_t)y * W;
+    refineFrom(r, raw, cf, x, D, minD, maxD, base, valid);
+    if (d0 == 0) labels[p] = labelWord(base, valid, im);
     const int lo = imax(minD, x - W + 1), hi = imin(maxD, x);
     int v[8];
 #pragma unroll
@@ -232,7 +322,10 @@ __global__ __launch_bounds__(256) void kCensusLocal(const uint2* desc, RefineIn 
         const int d = d0 + i;
         const int real = base + d;
         const bool ok = valid && d < D && real >= lo && real <= hi;
-        v[i] = ok ? __popc(cl ^ R[x - real].x) : kMax;
+        // ok => 0 <= x - real < W and x - real in the staged span.
+        const int x2 = ok ? x - real : (STAGED ? xb : 0);
+        const uint32_t cr = STAGED ? s_r[x2 - xb] : R[x2];
+        v[i] = ok ? __popc(cl ^ cr) : kMax;
     }
     storeCost8(C + p * Dp + d0, v);
 }
@@ -387,44 +480,76 @@ __device__ __forceinline__ int reflect101(int i, int n)
     return i < 0 ? 0 : i;
 }
 
+// Block: 64 x 8 destination pixels of one image, two rows per thread; the
+// 131 x 19 source window they read (reflect-101 coordinates) is loaded into
+// registers with every load in flight at once, then staged in LDS.
+constexpr int kPyrRows = 8;
+
 __global__ __launch_bounds__(256) void kPyrDown(const uint8_t* srcA, int sstepA, const uint8_t* srcB,
                                                 int sstepB, size_t sStride, int sw, int sh,
                                                 uint8_t* dstA, int dstepA, uint8_t* dstB,
                                                 int dstepB, size_t dStride, int dw, int dh, int nv)
 {
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (x >= dw || y >= dh) return;
+    constexpr int TW = 131, TH = 2 * kPyrRows + 3, TILE = TW * TH, NL = (TILE + 255) / 256;
+    __shared__ uint8_t s_t[TH][TW + 1];
     const size_t pr = blockIdx.z / nv;              // pair
     const int view = (int)(blockIdx.z - pr * nv);   // 0: A, 1: B
     const uint8_t* src = pairAt(view ? srcB : srcA, pr, sStride);
     const int sstep = view ? sstepB : sstepA;
-    int cols[5];
+    const int x0 = blockIdx.x * 64, y0 = blockIdx.y * kPyrRows;
+    uint32_t v[NL];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) cols[k] = reflect101(2 * x + k - 2, sw);
-    constexpr int w[5] = {1, 4, 6, 4, 1};
-    int sum = 0;
-#pragma unroll
-    for (int ky = 0; ky < 5; ++ky) {
-        const uint8_t* row = src + (size_t)reflect101(2 * y + ky - 2, sh) * sstep;
-        const int rs = row[cols[0]] + 4 * row[cols[1]] + 6 * row[cols[2]] + 4 * row[cols[3]] +
-                       row[cols[4]];
-        sum += w[ky] * rs;
+    for (int k = 0; k < NL; ++k) {
+        const int i = threadIdx.x + k * 256;
+        const int ty = i / TW, tx = i - ty * TW;
+        if (i < TILE)
+            v[k] = src[(size_t)reflect101(2 * y0 - 2 + ty, sh) * sstep + reflect101(2 * x0 - 2 + tx, sw)];
     }
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+        const int i = threadIdx.x + k * 256;
+        const int ty = i / TW, tx = i - ty * TW;
+        if (i < TILE) s_t[ty][tx] = (uint8_t)v[k];
+    }
+    __syncthreads();
+    const int lx = threadIdx.x & 63;
+    const int x = x0 + lx;
+    if (x >= dw) return;
     uint8_t* dst = pairAt(view ? dstB : dstA, pr, dStride);
     const int dstep = view ? dstepB : dstepA;
-    dst[(size_t)y * dstep + x] = (uint8_t)((sum + 128) >> 8);
+    constexpr int w[5] = {1, 4, 6, 4, 1};
+#pragma unroll
+    for (int rr = 0; rr < 2; ++rr) {
+        const int ly = (threadIdx.x >> 6) + 4 * rr;
+        const int y = y0 + ly;
+        if (y >= dh) break;
+        int sum = 0;
+#pragma unroll
+        for (int ky = 0; ky < 5; ++ky) {
+            const uint8_t* row = s_t[2 * ly + ky] + 2 * lx;
+            sum += w[ky] * (row[0] + 4 * row[1] + 6 * row[2] + 4 * row[3] + row[4]);
+        }
+        dst[(size_t)y * dstep + x] = (uint8_t)((sum + 128) >> 8);
+    }
 }
 
 } // namespace
 
 hipError_t launchPrep(const LevelCall& c, Workspace& ws, hipStream_t s)
 {
-    dim3 grid((c.W + 63) / 64, (c.H + 3) / 4, 2 * c.batch.n);
-    kPrep<<<grid, 256, 0, s>>>(c.left, c.lstep, c.right, c.rstep, ws.desc, ws.keys, c.W, c.H,
-                               c.cost == 1, c.cap, c.batch);
+    if (c.cost == 1)
+        kPrepCensus<<<dim3((c.W + 63) / 64, (c.H + kCenRows - 1) / kCenRows, 2 * c.batch.n), 256,
+                      0, s>>>(c.left, c.lstep, c.right, c.rstep,
+                                         reinterpret_cast<uint32_t*>(ws.desc), ws.keys, c.W, c.H,
+                                         c.batch);
+    else
+        kPrepBT<<<dim3((c.W + 63) / 64, (c.H + 3) / 4, 2 * c.batch.n), 256, 0, s>>>(c.left, c.lstep, c.right, c.rstep, ws.desc, ws.keys, c.W,
+                                     c.H, c.cap, c.batch);
     return hipGetLastError();
 }
+
+// Right census words a refinement block may stage in LDS (16 KB).
+constexpr long kMaxStaged = 4096;
 
 hipError_t launchCost(const LevelCall& c, Workspace& ws, hipStream_t s)
 {
@@ -434,9 +559,15 @@ hipError_t launchCost(const LevelCall& c, Workspace& ws, hipStream_t s)
     if (c.mode == kRefine) {
         RefineIn r{c.pdisp, c.pconf, c.pdpitch, c.pcpitch, c.PW, c.PH, c.minConf};
         dim3 grid((c.W + npx - 1) / npx, c.H, c.batch.n);
-        if (c.cost == 1)
-            kCensusLocal<<<grid, 256, 0, s>>>(ws.desc, r, ws.labels, c.left, c.lstep, ws.C, c.W,
-                                              c.H, c.D, c.Dp, c.minD, c.maxD, c.batch);
+        const uint32_t* cen = reinterpret_cast<const uint32_t*>(ws.desc);
+        const long span = (long)npx + c.maxD - c.minD;  // staged right columns
+        if (c.cost == 1 && span <= kMaxStaged)
+            kCensusLocal<true><<<grid, 256, (size_t)span * 4, s>>>(cen, r, ws.labels, c.left,
+                                                                   c.lstep, ws.C, c.W, c.H, c.D,
+                                                                   c.Dp, c.minD, c.maxD, c.batch);
+        else if (c.cost == 1)
+            kCensusLocal<false><<<grid, 256, 0, s>>>(cen, r, ws.labels, c.left, c.lstep, ws.C, c.W,
+                                                     c.H, c.D, c.Dp, c.minD, c.maxD, c.batch);
         else
             kBTLocal<<<grid, 256, 0, s>>>(ws.desc, r, ws.labels, c.left, c.lstep, ws.C, c.W, c.H,
                                           c.D, c.Dp, c.minD, c.maxD, half, c.batch);
@@ -445,8 +576,9 @@ hipError_t launchCost(const LevelCall& c, Workspace& ws, hipStream_t s)
     if (c.cost == 1) {
         dim3 grid((c.W + npx - 1) / npx, c.H, c.batch.n);
         const size_t lds = (size_t)(npx + c.Dp - 1) * sizeof(uint32_t);
-        kCensusCost<<<grid, 256, lds, s>>>(ws.desc, ws.C, c.W, c.H, c.D, c.Dp, c.minD, c.maxD,
-                                           c.mode == kCoarse, c.batch.ws);
+        kCensusCost<<<grid, 256, lds, s>>>(reinterpret_cast<const uint32_t*>(ws.desc), ws.C, c.W,
+                                           c.H, c.D, c.Dp, c.minD, c.maxD, c.mode == kCoarse,
+                                           c.batch.ws);
         return hipGetLastError();
     }
     const int dc = c.Dp < kChunkD ? c.Dp : kChunkD;
@@ -468,7 +600,7 @@ hipError_t launchPyrDownBatch(const uint8_t* srcA, int sstepA, const uint8_t* sr
                               hipStream_t stream)
 {
     const int nv = srcB ? 2 : 1;
-    dim3 grid((dw + 63) / 64, (dh + 3) / 4, nv * n);
+    dim3 grid((dw + 63) / 64, (dh + kPyrRows - 1) / kPyrRows, nv * n);
     kPyrDown<<<grid, 256, 0, stream>>>(srcA, sstepA, srcB, sstepB, sStride, sw, sh, dstA, dstepA,
                                        dstB, dstepB, dStride, dw, dh, nv);
     return hipGetLastError();

@@ -17,7 +17,9 @@
 namespace pyr {
 namespace {
 
-constexpr int kTile = 16;
+// Block: 64 x 8 pixels, two rows per thread (row segments of 64 pixels give
+// whole-line stores for every output).
+constexpr int kPostW = 64, kPostH = 8;
 
 struct PostArgs {
     const int16_t* disp;
@@ -32,24 +34,41 @@ struct PostArgs {
     Batch batch;
 };
 
-__device__ __forceinline__ int disp2At(const PostArgs& a, int y, int x2, int invalid)
+// The two right-view keys the LR check of disparity d at (x, y) reads (columns
+// x - (d >> 4) and x - ((d + 15) >> 4)); none when the check is off, d is
+// invalid or the column is outside the image.  Loads only, so that a thread
+// can have the keys of several pixels in flight at once.
+struct LrProbe {
+    uint32_t ka, kb;
+};
+
+__device__ __forceinline__ LrProbe lrLoad(const PostArgs& a, int x, int y, int d, int invalid)
 {
-    const uint32_t k = a.keys[(size_t)y * a.W + x2];
+    LrProbe pr{0xFFFFFFFFu, 0xFFFFFFFFu};
+    if (!a.lrcheck || d == invalid) return pr;
+    const int xa = x - (d >> 4), xb = x - ((d + 15) >> 4);
+    const uint32_t* row = a.keys + (size_t)y * a.W;
+    if (xa >= 0 && xa < a.W) pr.ka = row[xa];
+    if (xb >= 0 && xb < a.W) pr.kb = row[xb];
+    return pr;
+}
+
+__device__ __forceinline__ int disp2Of(uint32_t k, int x2, int invalid)
+{
     if (k == 0xFFFFFFFFu) return invalid;
     const int xw = 0xFFFF - (int)(k & 0xFFFFu);
     return (int)(int16_t)(xw - x2);
 }
 
-__device__ __forceinline__ int checked(const PostArgs& a, int x, int y)
+// d after the LR check, from the keys lrLoad read.
+__device__ __forceinline__ int lrDecide(const PostArgs& a, int x, int d, LrProbe pr, int invalid)
 {
-    const int invalid = (int)(int16_t)((a.minD - 1) * 16);
-    const int d = a.disp[(size_t)y * a.W + x];
     if (!a.lrcheck || d == invalid) return d;
     const int d0 = d >> 4, d1 = (d + 15) >> 4;
     const int xa = x - d0, xb = x - d1;
     const bool ina = xa >= 0 && xa < a.W, inb = xb >= 0 && xb < a.W;
-    const int ra = ina ? disp2At(a, y, xa, invalid) : 0;
-    const int rb = inb ? disp2At(a, y, xb, invalid) : 0;
+    const int ra = ina ? disp2Of(pr.ka, xa, invalid) : 0;
+    const int rb = inb ? disp2Of(pr.kb, xb, invalid) : 0;
     if (a.strict) {
         const bool oka = ina && ra >= a.minD && abs(ra - d0) <= a.maxDiff;
         const bool okb = inb && rb >= a.minD && abs(rb - d1) <= a.maxDiff;
@@ -84,6 +103,7 @@ __device__ __forceinline__ int median9(int p[9])
 
 __global__ __launch_bounds__(256) void kPost(PostArgs args)
 {
+    constexpr int HW = kPostW + 2, HALO = HW * (kPostH + 2), NH = (HALO + 255) / 256;
     // This pair's buffers (pair = blockIdx.z).
     PostArgs a = args;
     const size_t pr = blockIdx.z;
@@ -93,63 +113,102 @@ __global__ __launch_bounds__(256) void kPost(PostArgs args)
     a.out16 = pairAt(a.out16, pr, a.batch.out16);
     a.out32 = pairAt(a.out32, pr, a.batch.out32);
     a.outc = pairAt(a.outc, pr, a.batch.outc);
-    __shared__ int16_t s_d[kTile + 2][kTile + 2];
-    const int tx = threadIdx.x & (kTile - 1), ty = threadIdx.x / kTile;
-    const int bx = blockIdx.x * kTile, by = blockIdx.y * kTile;
+    __shared__ int16_t s_d[kPostH + 2][HW];
+    const int invalid = (int)(int16_t)((a.minD - 1) * 16);
+    const int lx = threadIdx.x & 63, ly0 = threadIdx.x >> 6;
+    const int bx = blockIdx.x * kPostW, by = blockIdx.y * kPostH;
+    const int x = bx + lx;
+    const int xc = imin(x, a.W - 1);
+    // Own pixels: raw disparity and confidence.
+    int raw[2];
+    float cf0[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const int yc = imin(by + ly0 + 4 * r, a.H - 1);
+        raw[r] = a.disp[(size_t)yc * a.W + xc];
+        cf0[r] = a.conf[(size_t)yc * a.W + xc];
+    }
+    int chk[2];
     if (a.median) {
-        for (int i = threadIdx.x; i < (kTile + 2) * (kTile + 2); i += 256) {
-            const int ly = i / (kTile + 2), lx = i - ly * (kTile + 2);
-            const int gx = bx + lx - 1, gy = by + ly - 1;
-            if (gx >= 0 && gx < a.W && gy >= 0 && gy < a.H) s_d[ly][lx] = (int16_t)checked(a, gx, gy);
+        // Checked disparities of the tile and its one-pixel ring (in-image cells only).
+        int hd[NH];
+        int hx[NH], hy[NH];
+#pragma unroll
+        for (int k = 0; k < NH; ++k) {
+            const int i = threadIdx.x + k * 256;
+            const int ty = i / HW, tx = i - ty * HW;
+            hx[k] = bx + tx - 1;
+            hy[k] = by + ty - 1;
+            const bool in = i < HALO && hx[k] >= 0 && hx[k] < a.W && hy[k] >= 0 && hy[k] < a.H;
+            hd[k] = in ? a.disp[(size_t)hy[k] * a.W + hx[k]] : invalid;
+            if (!in) hx[k] = -1;
+        }
+        LrProbe pb[NH];
+#pragma unroll
+        for (int k = 0; k < NH; ++k)
+            pb[k] = hx[k] >= 0 ? lrLoad(a, hx[k], hy[k], hd[k], invalid) : LrProbe{0xFFFFFFFFu, 0xFFFFFFFFu};
+#pragma unroll
+        for (int k = 0; k < NH; ++k) {
+            const int i = threadIdx.x + k * 256;
+            const int ty = i / HW, tx = i - ty * HW;
+            if (hx[k] >= 0) s_d[ty][tx] = (int16_t)lrDecide(a, hx[k], hd[k], pb[k], invalid);
         }
         __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 2; ++r) chk[r] = s_d[ly0 + 4 * r + 1][lx + 1];
+    } else {
+        LrProbe pb[2];
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+            pb[r] = lrLoad(a, xc, imin(by + ly0 + 4 * r, a.H - 1), raw[r], invalid);
+#pragma unroll
+        for (int r = 0; r < 2; ++r) chk[r] = lrDecide(a, xc, raw[r], pb[r], invalid);
     }
-    const int x = bx + tx, y = by + ty;
-    if (x >= a.W || y >= a.H) return;
-    const size_t p = (size_t)y * a.W + x;
-    const int invalid = (int)(int16_t)((a.minD - 1) * 16);
-    const int raw = a.disp[p];
-    const int chk = a.median ? s_d[ty + 1][tx + 1] : checked(a, x, y);
-    float cf = (chk == raw) ? a.conf[p] : 0.0f;
-    int fin = chk;
-    if (a.median) {
-        int v[9];
-        int n = 0;
+    if (x >= a.W) return;
 #pragma unroll
-        for (int dy = -1; dy <= 1; ++dy)
+    for (int r = 0; r < 2; ++r) {
+        const int ly = ly0 + 4 * r;
+        const int y = by + ly;
+        if (y >= a.H) break;
+        float cf = (chk[r] == raw[r]) ? cf0[r] : 0.0f;
+        int fin = chk[r];
+        if (a.median) {
+            int v[9];
+            int n = 0;
 #pragma unroll
-            for (int dx = -1; dx <= 1; ++dx) {
-                const bool in = x + dx >= 0 && x + dx < a.W && y + dy >= 0 && y + dy < a.H;
-                n += in;
-            }
-        const int nLow = 4 - n / 2;
-        int seenOut = 0;
-        int idx = 0;
+            for (int dy = -1; dy <= 1; ++dy)
 #pragma unroll
-        for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx)
+                    n += x + dx >= 0 && x + dx < a.W && y + dy >= 0 && y + dy < a.H;
+            const int nLow = 4 - n / 2;
+            int seenOut = 0;
+            int idx = 0;
 #pragma unroll
-            for (int dx = -1; dx <= 1; ++dx) {
-                const bool in = x + dx >= 0 && x + dx < a.W && y + dy >= 0 && y + dy < a.H;
-                int val;
-                if (in) {
-                    val = s_d[ty + 1 + dy][tx + 1 + dx];
-                } else {
-                    val = seenOut < nLow ? -100000 : 100000;
-                    ++seenOut;
+            for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const bool in = x + dx >= 0 && x + dx < a.W && y + dy >= 0 && y + dy < a.H;
+                    int val;
+                    if (in) {
+                        val = s_d[ly + 1 + dy][lx + 1 + dx];
+                    } else {
+                        val = seenOut < nLow ? -100000 : 100000;
+                        ++seenOut;
+                    }
+                    v[idx++] = val;
                 }
-                v[idx++] = val;
-            }
-        fin = median9(v);
-        if (fin != chk) cf = 0.0f;
-    }
-    if (a.out16) *reinterpret_cast<int16_t*>((char*)a.out16 + (size_t)y * a.s16 + (size_t)x * 2) = (int16_t)fin;
-    if (a.out32 && a.outc) {
-        const float d = (float)fin / 16.0f;
-        const bool ok = fin != invalid && d > 0.0f;
-        *reinterpret_cast<float*>((char*)a.out32 + (size_t)y * a.s32 + (size_t)x * 4) = ok ? d : 0.0f;
-        *reinterpret_cast<float*>((char*)a.outc + (size_t)y * a.sc + (size_t)x * 4) = ok ? cf : 0.0f;
-    } else if (a.outc && a.pyramid) {
-        *reinterpret_cast<float*>((char*)a.outc + (size_t)y * a.sc + (size_t)x * 4) = cf;
+            fin = median9(v);
+            if (fin != chk[r]) cf = 0.0f;
+        }
+        if (a.out16) *reinterpret_cast<int16_t*>((char*)a.out16 + (size_t)y * a.s16 + (size_t)x * 2) = (int16_t)fin;
+        if (a.out32 && a.outc) {
+            const float d = (float)fin / 16.0f;
+            const bool ok = fin != invalid && d > 0.0f;
+            *reinterpret_cast<float*>((char*)a.out32 + (size_t)y * a.s32 + (size_t)x * 4) = ok ? d : 0.0f;
+            *reinterpret_cast<float*>((char*)a.outc + (size_t)y * a.sc + (size_t)x * 4) = ok ? cf : 0.0f;
+        } else if (a.outc && a.pyramid) {
+            *reinterpret_cast<float*>((char*)a.outc + (size_t)y * a.sc + (size_t)x * 4) = cf;
+        }
     }
 }
 
@@ -177,7 +236,7 @@ hipError_t launchPost(const LevelCall& c, Workspace& ws, hipStream_t s)
     a.sc = c.sc;
     if (!a.out16 && !a.outc) return hipSuccess;
     a.batch = c.batch;
-    dim3 grid((c.W + kTile - 1) / kTile, (c.H + kTile - 1) / kTile, c.batch.n);
+    dim3 grid((c.W + kPostW - 1) / kPostW, (c.H + kPostH - 1) / kPostH, c.batch.n);
     kPost<<<grid, 256, 0, s>>>(a);
     return hipGetLastError();
 }

@@ -120,6 +120,254 @@ def derive_traffic(bench_json, fetch_csv, write_csv, cal_fetch_csv, cal_write_cs
     return res
 
 
+def _event_ms(torch, fn, reps, warm=3):
+    """Mean device milliseconds of fn() over reps runs on the current stream."""
+    for _ in range(warm):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def bench_batched(P, torch, log, n=32, W=960, H=540, D=128):
+    """32 pairs of 960x540, D=128, 3-level census pyramid: one BatchMatcher launch
+    per stage (whole pyramid replayed as one HIP graph) against the same pairs one
+    at a time through StereoMatcher; outputs compared bit for bit."""
+    from pyrsgm_amd import synth
+    params = P.validate_params(P.Params(num_disparities=D, pyramid_levels=2,
+                                        cost_mode=P.CostMode.Census, p1=8, p2=32,
+                                        adaptive_p2=True))
+    log(f"[bench] batched: generating {n} pairs {W}x{H}...")
+    ps = [synth.config_pair("960x540_d128_census_l2", i) for i in range(n)]
+    L = torch.stack([torch.from_numpy(p.left) for p in ps]).cuda()
+    R = torch.stack([torch.from_numpy(p.right) for p in ps]).cuda()
+    d16 = torch.empty((n, H, W), dtype=torch.int16, device="cuda")
+    d32 = torch.empty((n, H, W), dtype=torch.float32, device="cuda")
+    cf = torch.empty((n, H, W), dtype=torch.float32, device="cuda")
+    m = P.StereoMatcher(W, H, params)
+
+    def per_pair():
+        for i in range(n):
+            m.compute(L[i], R[i], d16[i], d32[i], cf[i])
+
+    t_pp = _event_ms(torch, per_pair, reps=5)
+    ref = (d16.clone(), d32.clone(), cf.clone())
+    m.close()
+    d16.fill_(0)
+    bm = P.BatchMatcher(W, H, n, params)
+    t_eager = _event_ms(torch, lambda: bm.compute(L, R, d16, d32, cf), reps=10)
+    bm.capture(L, R, d16, d32, cf)
+    d16.fill_(0)
+    t_graph = _event_ms(torch, bm.launch, reps=20)
+    same = bool((d16 == ref[0]).all()) and bool((d32 == ref[1]).all()) and bool((cf == ref[2]).all())
+    mem = bm.device_memory_bytes()
+    bm.close()
+    out = {"workload": f"{n}x{W}x{H}_d{D}_census_l2", "pairs": n,
+           "pairs_per_s": round(n / t_graph * 1000.0, 1),
+           "ms_per_batch": round(t_graph, 4), "eager_ms_per_batch": round(t_eager, 4),
+           "per_pair_pairs_per_s": round(n / t_pp * 1000.0, 1),
+           "per_pair_ms": round(t_pp / n, 4),
+           "speedup_vs_per_pair": round(t_pp / t_graph, 3),
+           "bit_identical_to_per_pair": same,
+           "device_bytes": mem}
+    log(f"[bench] batched: {out}")
+    return out
+
+
+def _mixed_requests(rng, n, kinds, pool):
+    """n requests (key, pair) drawn from kinds; pair images from the per-size pool."""
+    reqs = []
+    for j in range(n):
+        k = kinds[int(rng.integers(len(kinds)))]
+        reqs.append((k, pool[(k["w"], k["h"])][j % len(pool[(k["w"], k["h"])])]))
+    return reqs
+
+
+def _params_for(P, k):
+    census = k["cost"] == "census"
+    return P.Params(num_disparities=k["D"], pyramid_levels=k["levels"],
+                    cost_mode=P.CostMode.Census if census else P.CostMode.BirchfieldTomasi,
+                    p1=8 if census else 300, p2=32 if census else 2400,
+                    median_filter=k.get("median", True), adaptive_p2=census)
+
+
+def _sequential_service_ms(P, torch, reqs, dev_pairs, outs):
+    """Device time of each request alone through StereoMatcher (one matcher per
+    key, created beforehand), run back to back in arrival order."""
+    matchers = {}
+    for k, _ in reqs:
+        key = tuple(sorted(k.items()))
+        if key not in matchers:
+            matchers[key] = P.StereoMatcher(k["w"], k["h"], _params_for(P, k))
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in reqs]
+    for rep in range(2):  # the first pass warms every matcher up
+        for j, (k, pair) in enumerate(reqs):
+            m = matchers[tuple(sorted(k.items()))]
+            Ld, Rd = dev_pairs[id(pair)]
+            ev[j][0].record()
+            m.compute(Ld, Rd, outs[j], None, None)
+            ev[j][1].record()
+        torch.cuda.synchronize()
+    svc = [a.elapsed_time(b) for a, b in ev]
+    for m in matchers.values():
+        m.close()
+    return svc
+
+
+def _fifo(arrivals_ms, service_ms):
+    """Latencies of a single FIFO server with the given service times."""
+    t, lat = 0.0, []
+    for a, s in zip(arrivals_ms, service_ms):
+        t = max(t, a) + s
+        lat.append(t - a)
+    return lat
+
+
+def _pct(v, p):
+    v = sorted(v)
+    return v[min(len(v) - 1, max(0, int(-(-p * len(v) // 1)) - 1))] if v else None
+
+
+def _pool(P, sizes, per_size, seed0, log):
+    from pyrsgm_amd import synth
+    import torch
+    pool, dev_pairs = {}, {}
+    for (w, h, dmax) in sizes:
+        pool[(w, h)] = []
+        for i in range(per_size):
+            p = synth.make_pair(w, h, dmax - 1, seed=seed0 + 10 * len(pool) + i)
+            pool[(w, h)].append(p)
+            dev_pairs[id(p)] = (torch.from_numpy(p.left).cuda(), torch.from_numpy(p.right).cuda())
+    log(f"[bench] pool: {sum(len(v) for v in pool.values())} pairs")
+    return pool, dev_pairs
+
+
+def _serve_burst(P, torch, server, reqs, dev_pairs, outs):
+    """Submit every request at once; wall pairs/s from first submit to last completion."""
+    torch.cuda.synchronize()
+    t0 = P.Server.now_ns()
+    tickets = []
+    for j, (k, pair) in enumerate(reqs):
+        Ld, Rd = dev_pairs[id(pair)]
+        tickets.append(server.submit(Ld, Rd, _params_for(P, k), disp16=outs[j], arrival_ns=t0))
+    res = [server.wait(t) for t in tickets]
+    t1 = max(r.done_ns for r in res)
+    return res, (t1 - t0) * 1e-9
+
+
+def bench_uneven(P, torch, log, n=64, seed=2000):
+    """64 pairs drawn uniformly from {640x480, 960x540, 1280x720, 1920x1080} x
+    D {64,128,256} x pyramid_levels {0..3} x {BT, census} x median on/off, all
+    submitted at once to the serving loop (lanes of different keys overlap on
+    the device; same-key pairs share launches), against the same pairs one at a
+    time through StereoMatcher.  disp16 only, as in the reference bench."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    sizes = [(640, 480, 256), (960, 540, 256), (1280, 720, 256), (1920, 1080, 256)]
+    kinds = []
+    for (w, h, _) in sizes:
+        for D in (64, 128, 256):
+            for lv in (0, 1, 2, 3):
+                for cost in ("bt", "census"):
+                    for med in (True, False):
+                        kinds.append({"w": w, "h": h, "D": D, "levels": lv, "cost": cost,
+                                      "median": med})
+    pool, dev_pairs = _pool(P, sizes, 2, seed, log)
+    reqs = _mixed_requests(rng, n, kinds, pool)
+    outs = [torch.empty((k["h"], k["w"]), dtype=torch.int16, device="cuda") for k, _ in reqs]
+    svc = _sequential_service_ms(P, torch, reqs, dev_pairs, outs)
+    ref = [o.clone() for o in outs]
+    server = P.Server(max_batch=16)
+    _serve_burst(P, torch, server, reqs, dev_pairs, outs)  # lanes created, graphs captured
+    for o in outs:
+        o.fill_(0)
+    res, span = _serve_burst(P, torch, server, reqs, dev_pairs, outs)
+    same = all(bool((a == b).all()) for a, b in zip(outs, ref))
+    st = server.stats()
+    server.close()
+    seq_pps = n / (sum(svc) * 1e-3)
+    out = {"workload": f"{n} pairs, sizes 640x480..1920x1080, D 64/128/256, levels 0-3, "
+                       "BT/census, median on/off", "pairs": n,
+           "distinct_keys": len({tuple(sorted(k.items())) for k, _ in reqs}),
+           "pairs_per_s": round(n / span, 1),
+           "per_pair_pairs_per_s": round(seq_pps, 1),
+           "speedup_vs_per_pair": round(n / span / seq_pps, 3),
+           "bit_identical_to_per_pair": same,
+           "launches": st.launches, "mean_batch": round(st.mean_batch, 2)}
+    log(f"[bench] uneven: {out}")
+    return out
+
+
+def bench_serving(P, torch, log, nreq=400, seed=3000):
+    """Open-loop Poisson stream of pairs sized {640x480, 1280x720, 1920x1080,
+    2560x1440, 3840x2160} (D 64/128/256/256/512; pyramid_levels 0/1/2/2/3;
+    BT or census per request) through the serving loop, at arrival rates swept
+    as fractions of its measured capacity.  The same stream served sequentially
+    through StereoMatcher is the comparison: its latencies come from each
+    request's measured device time in a FIFO queue with the same arrivals."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    sizes = [(640, 480, 64), (1280, 720, 128), (1920, 1080, 256), (2560, 1440, 256),
+             (3840, 2160, 512)]
+    levels = {640: 0, 1280: 1, 1920: 2, 2560: 2, 3840: 3}
+    kinds = [{"w": w, "h": h, "D": d, "levels": levels[w], "cost": c, "median": True}
+             for (w, h, d) in sizes for c in ("bt", "census")]
+    pool, dev_pairs = _pool(P, sizes, 2, seed, log)
+    reqs = _mixed_requests(rng, nreq, kinds, pool)
+    outs = [torch.empty((k["h"], k["w"]), dtype=torch.int16, device="cuda") for k, _ in reqs]
+    svc = _sequential_service_ms(P, torch, reqs, dev_pairs, outs)
+    ref = [o.clone() for o in outs]
+    seq_capacity = nreq / (sum(svc) * 1e-3)
+    server = P.Server(max_batch=16)
+    _serve_burst(P, torch, server, reqs, dev_pairs, outs)  # warm: lanes + graphs
+    res, span = _serve_burst(P, torch, server, reqs, dev_pairs, outs)
+    capacity = nreq / span
+    same = all(bool((a == b).all()) for a, b in zip(outs, ref))
+    log(f"[bench] serving capacity {capacity:.1f} pairs/s (sequential {seq_capacity:.1f}); "
+        f"same as alone: {same}")
+    rates = []
+    for frac in (0.5, 0.8, 0.95):
+        lam = frac * capacity
+        gaps = rng.exponential(1.0 / lam, size=nreq)
+        arr = np.cumsum(gaps)
+        server.reset_stats()
+        torch.cuda.synchronize()
+        t0 = P.Server.now_ns()
+        tickets = []
+        for j, (k, pair) in enumerate(reqs):
+            due = t0 + int(arr[j] * 1e9)
+            while P.Server.now_ns() < due:
+                pass
+            Ld, Rd = dev_pairs[id(pair)]
+            tickets.append(server.submit(Ld, Rd, _params_for(P, k), disp16=outs[j], arrival_ns=due))
+        res = [server.wait(t) for t in tickets]
+        st = server.stats()
+        lat = [r.latency_ms for r in res]
+        seq_lat = _fifo(list(arr * 1e3), svc)
+        rates.append({"offered_pairs_per_s": round(lam, 1),
+                      "pairs_per_s": round(st.pairs_per_s, 1),
+                      "p50_ms": round(_pct(lat, 0.5), 3), "p99_ms": round(_pct(lat, 0.99), 3),
+                      "mean_batch": round(st.mean_batch, 2),
+                      "sequential_p50_ms": round(_pct(seq_lat, 0.5), 3),
+                      "sequential_p99_ms": round(_pct(seq_lat, 0.99), 3)})
+        log(f"[bench] serving at {lam:.0f}/s: {rates[-1]}")
+    st = server.stats()
+    server.close()
+    return {"workload": "Poisson stream, sizes 640x480/1280x720/1920x1080/2560x1440/3840x2160, "
+                        "D 64/128/256/256/512, levels 0/1/2/2/3, BT or census, disp16 out",
+            "requests_per_rate": nreq, "capacity_pairs_per_s": round(capacity, 1),
+            "sequential_capacity_pairs_per_s": round(seq_capacity, 1),
+            "speedup_vs_sequential": round(capacity / seq_capacity, 3),
+            "bit_identical_to_alone": same, "rates": rates,
+            "peak_device_bytes": st.peak_bytes, "lanes": st.lanes}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -141,6 +389,11 @@ def main() -> int:
                          "(and of a --copy-calibration run) into per-launch HBM bytes, "
                          "printed as JSON")
     ap.add_argument("--calib-mib", type=int, default=512)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched leg")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the uneven-batch and serving legs")
+    ap.add_argument("--serve-requests", type=int, default=400,
+                    help="requests per arrival rate in the serving leg")
     args = ap.parse_args()
     if args.derive_traffic:
         print(json.dumps(derive_traffic(*args.derive_traffic, args.calib_mib << 20), indent=1))
@@ -296,6 +549,12 @@ def main() -> int:
         }
         log(f"[bench] cpu {c1 - c0:.2f} s/frame; parity {parity}")
 
+    batched = None if args.no_batch else bench_batched(P, torch, log)
+    uneven = serving = None
+    if not args.no_serve:
+        uneven = bench_uneven(P, torch, log)
+        serving = bench_serving(P, torch, log, args.serve_requests)
+
     valid = d16.cpu().numpy() > 0
     err = np.abs(d16.cpu().numpy() / 16.0 - pair.gt)[valid]
     out = {
@@ -323,6 +582,9 @@ def main() -> int:
         "parity_vs_oracle": parity,
         "accuracy": {"density": round(float(valid.mean()), 4),
                      "bad1": round(float((err > 1).mean()), 4) if err.size else None},
+        "batched": batched,
+        "uneven_batch": uneven,
+        "serving": serving,
         "kernels": kernels,
         "launch_sequence": seq,
     }

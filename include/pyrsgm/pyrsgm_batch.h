@@ -9,9 +9,10 @@
  * bit-identical to running the pairs one at a time through StereoMatcher.
  *
  * Pair i of a batched buffer lives at base + i * pair_stride (bytes), each pair
- * pitched by the given row pitch.  compute() allocates nothing.  capture()
- * records the whole pyramid for one set of buffers into a HIP graph, which
- * launch() replays with one call.
+ * pitched by the given row pitch.  compute() allocates nothing and may run
+ * fewer pairs than the matcher holds (the first `count`).  capture() records
+ * the whole pyramid for one set of buffers into a HIP graph, which launch()
+ * replays with one call.
  */
 #ifndef PYRSGM_BATCH_H
 #define PYRSGM_BATCH_H
@@ -45,13 +46,15 @@ public:
     BatchMatcher(const BatchMatcher&) = delete;
     BatchMatcher& operator=(const BatchMatcher&) = delete;
 
-    /// Enqueue all pairs on `stream`.  Null outputs are skipped as in StereoMatcher.
+    /// Enqueue the first `count` pairs (-1: all) on `stream`.  Null outputs are
+    /// skipped as in StereoMatcher.
     void compute(BatchBuffer<const uint8_t> left, BatchBuffer<const uint8_t> right,
                  BatchBuffer<int16_t> disp16, BatchBuffer<float> disp32, BatchBuffer<float> conf,
-                 hipStream_t stream = nullptr);
+                 hipStream_t stream = nullptr, int count = -1);
     /// Record compute() on these buffers into a graph (replaces a previous one).
     void capture(BatchBuffer<const uint8_t> left, BatchBuffer<const uint8_t> right,
-                 BatchBuffer<int16_t> disp16, BatchBuffer<float> disp32, BatchBuffer<float> conf);
+                 BatchBuffer<int16_t> disp16, BatchBuffer<float> disp32, BatchBuffer<float> conf,
+                 int count = -1);
     /// Replay the captured graph on `stream`.  Throws if nothing was captured.
     void launch(hipStream_t stream = nullptr);
 
@@ -62,6 +65,8 @@ public:
     int numLevels() const;
     LevelInfo levelInfo(int level) const;
     size_t deviceMemoryBytes() const;    ///< device memory owned by the matcher
+    /// Device memory a matcher of these arguments would allocate (validates them).
+    static size_t requiredBytes(int width, int height, int batch, const Params& params);
 
 private:
     struct Impl;
@@ -84,16 +89,19 @@ int pyrsgm_batch_compute(void* matcher,
                          short* d_disp16, size_t disp16_pitch, size_t disp16_stride,
                          float* d_disp32, size_t disp32_pitch, size_t disp32_stride,
                          float* d_conf, size_t conf_pitch, size_t conf_stride,
-                         hipStream_t stream, char* err, int errlen);
+                         int count, hipStream_t stream, char* err, int errlen);
 int pyrsgm_batch_capture(void* matcher,
                          const unsigned char* d_left, size_t left_pitch, size_t left_stride,
                          const unsigned char* d_right, size_t right_pitch, size_t right_stride,
                          short* d_disp16, size_t disp16_pitch, size_t disp16_stride,
                          float* d_disp32, size_t disp32_pitch, size_t disp32_stride,
                          float* d_conf, size_t conf_pitch, size_t conf_stride,
-                         char* err, int errlen);
+                         int count, char* err, int errlen);
 int pyrsgm_batch_launch(void* matcher, hipStream_t stream, char* err, int errlen);
 size_t pyrsgm_batch_device_bytes(void* matcher);
+/* BatchMatcher::requiredBytes; 0 on invalid arguments (message in err). */
+size_t pyrsgm_batch_required_bytes(int width, int height, int batch,
+                                   const pyrsgm_params_c* params, char* err, int errlen);
 
 #ifdef __cplusplus
 }

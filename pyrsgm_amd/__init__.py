@@ -107,6 +107,39 @@ def _from_c(s: _ParamsC) -> Params:
     return Params(**kw)
 
 
+class _ServeRequestC(C.Structure):
+    _fields_ = [("left", C.c_void_p), ("left_pitch", C.c_size_t),
+                ("right", C.c_void_p), ("right_pitch", C.c_size_t),
+                ("width", C.c_int), ("height", C.c_int),
+                ("disp16", C.c_void_p), ("disp16_pitch", C.c_size_t),
+                ("disp32", C.c_void_p), ("disp32_pitch", C.c_size_t),
+                ("conf", C.c_void_p), ("conf_pitch", C.c_size_t)]
+
+
+class ServeResult(C.Structure):
+    """pyrsgm_serve_result_c: a finished request (times in steady-clock ns)."""
+    _fields_ = [("ticket", C.c_ulonglong), ("status", C.c_int), ("batch_size", C.c_int),
+                ("arrival_ns", C.c_longlong), ("launch_ns", C.c_longlong),
+                ("done_ns", C.c_longlong)]
+
+    @property
+    def latency_ms(self) -> float:
+        return (self.done_ns - self.arrival_ns) * 1e-6
+
+
+class ServeStats(C.Structure):
+    """pyrsgm_serve_stats_c."""
+    _fields_ = [("submitted", C.c_ulonglong), ("completed", C.c_ulonglong),
+                ("failed", C.c_ulonglong), ("pairs_per_s", C.c_double), ("p50_ms", C.c_double),
+                ("p99_ms", C.c_double), ("max_ms", C.c_double), ("mean_ms", C.c_double),
+                ("mean_batch", C.c_double), ("resident_bytes", C.c_size_t),
+                ("peak_bytes", C.c_size_t), ("budget_bytes", C.c_size_t), ("lanes", C.c_int),
+                ("lanes_created", C.c_int), ("lanes_evicted", C.c_int), ("launches", C.c_int)]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 def lib() -> C.CDLL:
     """The loaded HIP library; raises if it has not been built."""
     global _lib
@@ -140,11 +173,29 @@ def lib() -> C.CDLL:
         L.pyrsgm_batch_create.restype = vp
         L.pyrsgm_batch_destroy.argtypes = [vp]
         L.pyrsgm_batch_destroy.restype = None
-        L.pyrsgm_batch_compute.argtypes = [vp] + [vp, sz, sz] * 5 + [vp, C.c_char_p, ci]
-        L.pyrsgm_batch_capture.argtypes = [vp] + [vp, sz, sz] * 5 + [C.c_char_p, ci]
+        L.pyrsgm_batch_compute.argtypes = [vp] + [vp, sz, sz] * 5 + [ci, vp, C.c_char_p, ci]
+        L.pyrsgm_batch_capture.argtypes = [vp] + [vp, sz, sz] * 5 + [ci, C.c_char_p, ci]
+        L.pyrsgm_batch_required_bytes.argtypes = [ci, ci, ci, C.POINTER(_ParamsC), C.c_char_p, ci]
+        L.pyrsgm_batch_required_bytes.restype = sz
         L.pyrsgm_batch_launch.argtypes = [vp, vp, C.c_char_p, ci]
         L.pyrsgm_batch_device_bytes.argtypes = [vp]
         L.pyrsgm_batch_device_bytes.restype = sz
+        ll, ull = C.c_longlong, C.c_ulonglong
+        L.pyrsgm_serve_create.argtypes = [sz, ci, ci, ci, C.c_char_p, ci]
+        L.pyrsgm_serve_create.restype = vp
+        L.pyrsgm_serve_destroy.argtypes = [vp]
+        L.pyrsgm_serve_destroy.restype = None
+        L.pyrsgm_serve_submit.argtypes = [vp, C.POINTER(_ServeRequestC), C.POINTER(_ParamsC), ll,
+                                          C.c_char_p, ci]
+        L.pyrsgm_serve_submit.restype = ll
+        L.pyrsgm_serve_poll.argtypes = [vp, ull, C.POINTER(ServeResult)]
+        L.pyrsgm_serve_wait.argtypes = [vp, ull, C.POINTER(ServeResult), C.c_char_p, ci]
+        L.pyrsgm_serve_drain.argtypes = [vp]
+        L.pyrsgm_serve_take.argtypes = [vp, C.POINTER(ServeResult), ci]
+        L.pyrsgm_serve_stats.argtypes = [vp, C.POINTER(ServeStats)]
+        L.pyrsgm_serve_reset_stats.argtypes = [vp]
+        L.pyrsgm_serve_reset_stats.restype = None
+        L.pyrsgm_serve_now_ns.restype = ll
         _lib = L
     return _lib
 
@@ -327,29 +378,42 @@ class BatchMatcher:
         except Exception:
             pass
 
-    def _args(self, left, right, disp16, disp32, conf):
+    @staticmethod
+    def required_bytes(width: int, height: int, batch: int, params: Params) -> int:
+        """Device memory a BatchMatcher of these arguments allocates."""
+        err = C.create_string_buffer(512)
+        n = lib().pyrsgm_batch_required_bytes(width, height, batch, C.byref(_to_c(params)), err, 512)
+        if n == 0:
+            raise ValueError(err.value.decode())
+        return n
+
+    def _args(self, left, right, disp16, disp32, conf, count):
+        count = self.batch if count is None else count
+        if not 1 <= count <= self.batch:
+            raise ValueError(f"pyrsgm: pair count {count} outside [1, {self.batch}]")
         for t in (left, right, disp16, disp32, conf):
-            if t is not None and (t.dim() != 3 or t.shape[0] != self.batch or
+            if t is not None and (t.dim() != 3 or t.shape[0] < count or t.shape[0] > self.batch or
                                   t.shape[1] != self.height or t.shape[2] != self.width):
-                raise ValueError(f"pyrsgm: expected [{self.batch}, {self.height}, {self.width}] "
-                                 f"tensors, got {tuple(t.shape)}")
+                raise ValueError(f"pyrsgm: expected [{count}..{self.batch}, {self.height}, "
+                                 f"{self.width}] tensors, got {tuple(t.shape)}")
         out = []
         for t in (left, right, disp16, disp32, conf):
             out += list(_bptr(t))
-        return out
+        return out + [count]
 
-    def compute(self, left, right, disp16=None, disp32=None, conf=None, stream=None):
+    def compute(self, left, right, disp16=None, disp32=None, conf=None, stream=None, count=None):
+        """The first `count` pairs (default: all) of [N, H, W] tensors."""
         import torch
         s = stream if stream is not None else torch.cuda.current_stream()
         err = C.create_string_buffer(512)
-        if lib().pyrsgm_batch_compute(self._h, *self._args(left, right, disp16, disp32, conf),
+        if lib().pyrsgm_batch_compute(self._h, *self._args(left, right, disp16, disp32, conf, count),
                                       s.cuda_stream, err, 512) != 0:
             raise RuntimeError(err.value.decode())
 
-    def capture(self, left, right, disp16=None, disp32=None, conf=None):
+    def capture(self, left, right, disp16=None, disp32=None, conf=None, count=None):
         """Record compute() on these tensors into a HIP graph; replay with launch()."""
         err = C.create_string_buffer(512)
-        if lib().pyrsgm_batch_capture(self._h, *self._args(left, right, disp16, disp32, conf),
+        if lib().pyrsgm_batch_capture(self._h, *self._args(left, right, disp16, disp32, conf, count),
                                       err, 512) != 0:
             raise RuntimeError(err.value.decode())
 
@@ -362,3 +426,95 @@ class BatchMatcher:
 
     def device_memory_bytes(self) -> int:
         return lib().pyrsgm_batch_device_bytes(self._h)
+
+
+class Server:
+    """pyrsgm::Server: the serving loop (include/pyrsgm/pyrsgm_serve.h).
+
+    Requests are 2-D device tensors (uint8 in; int16 / float32 out) that must
+    stay alive and unchanged until the request is finished."""
+
+    def __init__(self, memory_budget: int = 0, max_batch: int = 16, lanes_per_key: int = 2,
+                 use_graphs: bool = True):
+        err = C.create_string_buffer(512)
+        h = lib().pyrsgm_serve_create(memory_budget, max_batch, lanes_per_key, int(use_graphs),
+                                      err, 512)
+        if not h:
+            msg = err.value.decode()
+            raise (ValueError if "must be" in msg else RuntimeError)(msg)
+        self._h = h
+        self._keep = {}  # ticket -> tensors the device work reads or writes
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pyrsgm_serve_destroy(self._h)  # finishes everything submitted
+            self._h = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def now_ns() -> int:
+        return lib().pyrsgm_serve_now_ns()
+
+    def submit(self, left, right, params: Params, disp16=None, disp32=None, conf=None,
+               arrival_ns: int = 0) -> int:
+        """Queue one pair; returns its ticket.  arrival_ns (steady clock) defaults to now."""
+        h, w = left.shape
+        req = _ServeRequestC()
+        req.width, req.height = w, h
+        for name, t in (("left", left), ("right", right), ("disp16", disp16),
+                        ("disp32", disp32), ("conf", conf)):
+            ptr, pitch = _ptr(t)
+            if t is not None and tuple(t.shape) != (h, w):
+                raise ValueError(f"pyrsgm: {name} has shape {tuple(t.shape)}, expected {(h, w)}")
+            setattr(req, name, ptr)
+            setattr(req, name + "_pitch", pitch)
+        err = C.create_string_buffer(512)
+        t = lib().pyrsgm_serve_submit(self._h, C.byref(req), C.byref(_to_c(params)), arrival_ns,
+                                      err, 512)
+        if t < 0:
+            raise ValueError(err.value.decode())
+        self._keep[t] = (left, right, disp16, disp32, conf)
+        return t
+
+    def poll(self, ticket: int):
+        """The ServeResult once the ticket is finished, else None."""
+        out = ServeResult()
+        if lib().pyrsgm_serve_poll(self._h, ticket, C.byref(out)) == 1:
+            self._keep.pop(ticket, None)
+            return out
+        return None
+
+    def wait(self, ticket: int) -> ServeResult:
+        out = ServeResult()
+        err = C.create_string_buffer(512)
+        if lib().pyrsgm_serve_wait(self._h, ticket, C.byref(out), err, 512) != 0:
+            raise ValueError(err.value.decode())
+        self._keep.pop(ticket, None)
+        return out
+
+    def drain(self):
+        lib().pyrsgm_serve_drain(self._h)
+
+    def take(self, max_results: int = 4096) -> list:
+        """Finished results not yet polled (at most max_results)."""
+        arr = (ServeResult * max_results)()
+        n = lib().pyrsgm_serve_take(self._h, arr, max_results)
+        out = [arr[i] for i in range(max(n, 0))]
+        for r in out:
+            self._keep.pop(r.ticket, None)
+        return out
+
+    def stats(self) -> ServeStats:
+        out = ServeStats()
+        lib().pyrsgm_serve_stats(self._h, C.byref(out))
+        return out
+
+    def reset_stats(self):
+        lib().pyrsgm_serve_reset_stats(self._h)
+

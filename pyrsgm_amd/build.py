@@ -26,9 +26,11 @@ SOURCES = [
     "kernels_wta.hip",
     "kernels_cost.hip",
     "kernels_post.hip",
+    "kernels_io.hip",
     "capi.cpp",
     "stereo_matcher.cpp",
     "matcher_capi.cpp",
+    "server.cpp",
     "prof.cpp",
 ]
 

@@ -60,7 +60,8 @@ void releaseWorkspace(Workspace* ws)
 int ensureWorkspace(Workspace* ws, int W, int H, int D, int n)
 {
     if (!ws || n < 1) return -1;
-    if (ws->base && ws->W == W && ws->H == H && ws->D == D && ws->n == n) return 0;
+    // A workspace holding at least n slices of this geometry serves n pairs.
+    if (ws->base && ws->W == W && ws->H == H && ws->D == D && ws->n >= n) return 0;
     releaseWorkspace(ws);
     const Layout l = layoutFor(W, H, D);
     void* base = nullptr;

@@ -57,7 +57,8 @@ struct Workspace {
 };
 
 size_t workspaceBytes(int W, int H, int D);
-// (Re)allocates n slices of one level's buffers unless the geometry matches.
+// (Re)allocates n slices of one level's buffers unless the geometry matches
+// and at least n slices are there.
 int ensureWorkspace(Workspace* ws, int W, int H, int D, int n = 1);
 void releaseWorkspace(Workspace* ws);
 
@@ -151,6 +152,17 @@ hipError_t launchPyrDownBatch(const uint8_t* srcA, int sstepA, const uint8_t* sr
                               size_t sStride, int sw, int sh, uint8_t* dstA, int dstepA,
                               uint8_t* dstB, int dstepB, size_t dStride, int dw, int dh, int n,
                               hipStream_t stream);
+
+// One 2-D copy of a batched copy launch (pitches and row length in bytes).
+struct CopyDesc {
+    const uint8_t* src;
+    uint8_t* dst;
+    size_t spitch, dpitch;
+    int rowBytes, rows;
+};
+// `count` copies read from the device array `descs`, in one launch.
+hipError_t launchCopy2D(const CopyDesc* descs, int count, int maxRowBytes, int maxRows,
+                        hipStream_t stream);
 
 hipError_t launchPrep(const LevelCall& c, Workspace& ws, hipStream_t s);
 hipError_t launchCost(const LevelCall& c, Workspace& ws, hipStream_t s);

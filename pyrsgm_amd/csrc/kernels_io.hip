@@ -1,0 +1,41 @@
+// kernels_io.hip -- batched 2-D copies between caller buffers and a serving
+// lane's stacked staging buffers (one launch for any number of copies).
+#include "device_common.h"
+#include "internal.h"
+
+namespace pyr {
+namespace {
+
+// Block: 256 threads over 1024 bytes of one row of copy blockIdx.z.  Rows are
+// copied in 4-byte words when both ends, both pitches and the row length are
+// 4-byte aligned, else byte by byte.
+__global__ __launch_bounds__(256) void kCopy2D(const CopyDesc* descs)
+{
+    const CopyDesc d = descs[blockIdx.z];
+    const int y = blockIdx.y;
+    const int x = (blockIdx.x * 256 + threadIdx.x) * 4;
+    if (y >= d.rows || x >= d.rowBytes) return;
+    const uint8_t* src = d.src + (size_t)y * d.spitch;
+    uint8_t* dst = d.dst + (size_t)y * d.dpitch;
+    const bool words = (((uintptr_t)d.src | (uintptr_t)d.dst | d.spitch | d.dpitch |
+                         (size_t)d.rowBytes) & 3) == 0;
+    if (words) {
+        *reinterpret_cast<uint32_t*>(dst + x) = *reinterpret_cast<const uint32_t*>(src + x);
+    } else {
+        const int end = imin(x + 4, d.rowBytes);
+        for (int i = x; i < end; ++i) dst[i] = src[i];
+    }
+}
+
+} // namespace
+
+hipError_t launchCopy2D(const CopyDesc* descs, int count, int maxRowBytes, int maxRows,
+                        hipStream_t stream)
+{
+    if (count <= 0) return hipSuccess;
+    dim3 grid((unsigned)((maxRowBytes + 1023) / 1024), (unsigned)maxRows, (unsigned)count);
+    kCopy2D<<<grid, 256, 0, stream>>>(descs);
+    return hipGetLastError();
+}
+
+} // namespace pyr

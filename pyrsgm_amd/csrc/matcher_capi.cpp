@@ -9,6 +9,7 @@
 #include "pyrsgm/pyrsgm.h"
 #include "pyrsgm/pyrsgm_batch.h"
 #include "pyrsgm/pyrsgm_py.h"
+#include "pyrsgm/pyrsgm_serve.h"
 
 namespace {
 
@@ -186,14 +187,14 @@ int pyrsgm_batch_compute(void* matcher,
                          short* d_disp16, size_t disp16_pitch, size_t disp16_stride,
                          float* d_disp32, size_t disp32_pitch, size_t disp32_stride,
                          float* d_conf, size_t conf_pitch, size_t conf_stride,
-                         hipStream_t stream, char* err, int errlen)
+                         int count, hipStream_t stream, char* err, int errlen)
 {
     if (!matcher) return -1;
     try {
         static_cast<pyrsgm::BatchMatcher*>(matcher)->compute(
             {d_left, left_pitch, left_stride}, {d_right, right_pitch, right_stride},
             {d_disp16, disp16_pitch, disp16_stride}, {d_disp32, disp32_pitch, disp32_stride},
-            {d_conf, conf_pitch, conf_stride}, stream);
+            {d_conf, conf_pitch, conf_stride}, stream, count);
         return 0;
     } catch (const std::exception& e) {
         copyErr(e.what(), err, errlen);
@@ -207,14 +208,14 @@ int pyrsgm_batch_capture(void* matcher,
                          short* d_disp16, size_t disp16_pitch, size_t disp16_stride,
                          float* d_disp32, size_t disp32_pitch, size_t disp32_stride,
                          float* d_conf, size_t conf_pitch, size_t conf_stride,
-                         char* err, int errlen)
+                         int count, char* err, int errlen)
 {
     if (!matcher) return -1;
     try {
         static_cast<pyrsgm::BatchMatcher*>(matcher)->capture(
             {d_left, left_pitch, left_stride}, {d_right, right_pitch, right_stride},
             {d_disp16, disp16_pitch, disp16_stride}, {d_disp32, disp32_pitch, disp32_stride},
-            {d_conf, conf_pitch, conf_stride});
+            {d_conf, conf_pitch, conf_stride}, count);
         return 0;
     } catch (const std::exception& e) {
         copyErr(e.what(), err, errlen);
@@ -238,5 +239,109 @@ size_t pyrsgm_batch_device_bytes(void* matcher)
 {
     return matcher ? static_cast<pyrsgm::BatchMatcher*>(matcher)->deviceMemoryBytes() : 0;
 }
+
+size_t pyrsgm_batch_required_bytes(int width, int height, int batch,
+                                   const pyrsgm_params_c* params, char* err, int errlen)
+{
+    if (!params) return 0;
+    try {
+        return pyrsgm::BatchMatcher::requiredBytes(width, height, batch, toParams(*params));
+    } catch (const std::exception& e) {
+        copyErr(e.what(), err, errlen);
+    }
+    return 0;
+}
+
+// ---- Server ----------------------------------------------------------------
+void* pyrsgm_serve_create(size_t memory_budget, int max_batch, int lanes_per_key, int use_graphs,
+                          char* err, int errlen)
+{
+    try {
+        pyrsgm::ServeOptions o;
+        o.memory_budget = memory_budget;
+        o.max_batch = max_batch;
+        o.lanes_per_key = lanes_per_key;
+        o.use_graphs = use_graphs != 0;
+        return new pyrsgm::Server(o);
+    } catch (const std::exception& e) {
+        copyErr(e.what(), err, errlen);
+    }
+    return nullptr;
+}
+
+void pyrsgm_serve_destroy(void* server) { delete static_cast<pyrsgm::Server*>(server); }
+
+long long pyrsgm_serve_submit(void* server, const pyrsgm_serve_request_c* request,
+                              const pyrsgm_params_c* params, long long arrival_ns, char* err,
+                              int errlen)
+{
+    if (!server || !request || !params) return -1;
+    try {
+        pyrsgm::ServeRequest r;
+        r.left = request->left;
+        r.left_pitch = request->left_pitch;
+        r.right = request->right;
+        r.right_pitch = request->right_pitch;
+        r.width = request->width;
+        r.height = request->height;
+        r.disp16 = request->disp16;
+        r.disp16_pitch = request->disp16_pitch;
+        r.disp32 = request->disp32;
+        r.disp32_pitch = request->disp32_pitch;
+        r.conf = request->conf;
+        r.conf_pitch = request->conf_pitch;
+        r.params = toParams(*params);
+        return (long long)static_cast<pyrsgm::Server*>(server)->submit(r, arrival_ns);
+    } catch (const std::exception& e) {
+        copyErr(e.what(), err, errlen);
+    }
+    return -1;
+}
+
+int pyrsgm_serve_poll(void* server, unsigned long long ticket, pyrsgm_serve_result_c* out)
+{
+    if (!server) return -1;
+    return static_cast<pyrsgm::Server*>(server)->poll(ticket, out) ? 1 : 0;
+}
+
+int pyrsgm_serve_wait(void* server, unsigned long long ticket, pyrsgm_serve_result_c* out,
+                      char* err, int errlen)
+{
+    if (!server || !out) return -1;
+    try {
+        *out = static_cast<pyrsgm::Server*>(server)->wait(ticket);
+        return 0;
+    } catch (const std::exception& e) {
+        copyErr(e.what(), err, errlen);
+    }
+    return -1;
+}
+
+int pyrsgm_serve_drain(void* server)
+{
+    if (!server) return -1;
+    static_cast<pyrsgm::Server*>(server)->drain();
+    return 0;
+}
+
+int pyrsgm_serve_take(void* server, pyrsgm_serve_result_c* out, int max)
+{
+    if (!server || !out || max < 0) return -1;
+    return (int)static_cast<pyrsgm::Server*>(server)->takeResults(out, (size_t)max);
+}
+
+int pyrsgm_serve_stats(void* server, pyrsgm_serve_stats_c* out)
+{
+    if (!server || !out) return -1;
+    *out = static_cast<pyrsgm::Server*>(server)->stats();
+    return 0;
+}
+
+void pyrsgm_serve_reset_stats(void* server)
+{
+    if (server) static_cast<pyrsgm::Server*>(server)->resetStats();
+}
+
+long long pyrsgm_serve_now_ns(void) { return (long long)pyrsgm::Server::nowNs(); }
 
 } // extern "C"

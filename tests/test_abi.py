@@ -27,7 +27,7 @@ FAKE = C.c_void_p(0x1000)  # non-null, never dereferenced
 def declared_c_functions():
     """Names of the functions declared in the C headers (extern "C" blocks)."""
     names = set()
-    for h in ("pyrsgm_c.h", "pyrsgm_py.h", "pyrsgm_prof.h", "pyrsgm_batch.h"):
+    for h in ("pyrsgm_c.h", "pyrsgm_py.h", "pyrsgm_prof.h", "pyrsgm_batch.h", "pyrsgm_serve.h"):
         text = open(os.path.join(INC, h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         for m in re.finditer(r"\b(pyrsgm_[a-z0-9_]+)\s*\(", text):
@@ -68,6 +68,13 @@ def test_library_exports_the_cpp_api():
                 "pyrsgm::BatchMatcher::capture(",
                 "pyrsgm::BatchMatcher::launch(",
                 "pyrsgm::BatchMatcher::batch() const",
+                "pyrsgm::BatchMatcher::requiredBytes(int, int, int, pyrsgm::Params const&)",
+                "pyrsgm::Server::Server(pyrsgm::ServeOptions const&)",
+                "pyrsgm::Server::submit(pyrsgm::ServeRequest const&, long)",
+                "pyrsgm::Server::wait(unsigned long)",
+                "pyrsgm::Server::poll(unsigned long, pyrsgm_serve_result_c*)",
+                "pyrsgm::Server::drain()",
+                "pyrsgm::Server::stats() const",
                 "pyrsgm::version()"):
         assert sym in out, sym
 
@@ -80,6 +87,27 @@ def test_batch_rejects_bad_sizes_on_cpu():
         P.BatchMatcher(0, 32, 2, P.Params(num_disparities=64))
     with pytest.raises(ValueError):
         P.BatchMatcher(64, 32, 2, P.Params(p1=10, p2=5))
+
+
+def test_batch_required_bytes_on_cpu():
+    # The allocation formula needs no device: it grows with the pair count and
+    # validates its arguments.
+    p = P.Params(num_disparities=128, pyramid_levels=2, cost_mode=P.CostMode.Census)
+    one = P.BatchMatcher.required_bytes(960, 540, 1, p)
+    assert one > 960 * 540 * 16 * 2
+    assert P.BatchMatcher.required_bytes(960, 540, 4, p) >= 4 * one - 4 * 256 * 540
+    with pytest.raises(ValueError):
+        P.BatchMatcher.required_bytes(960, 540, 0, p)
+    with pytest.raises(ValueError):
+        P.BatchMatcher.required_bytes(960, 540, 1, P.Params(p1=10, p2=5))
+
+
+def test_server_rejects_bad_options_on_cpu():
+    # Option checks run before any device call.
+    with pytest.raises(ValueError):
+        P.Server(max_batch=0)
+    with pytest.raises(ValueError):
+        P.Server(lanes_per_key=0)
 
 
 def test_version():

@@ -130,3 +130,38 @@ def test_batch_graph_replays_match_eager():
             U.assert_same("replay 0 pair 1 vs oracle", g16[1].cpu().numpy(), ref.disp16)
     bm.close()
     eager.close()
+
+
+def test_batch_partial_counts():
+    """A matcher for 4 pairs run on its first 2, then all 4, then 1 (eager and
+    captured): every pair equals the single-pair result, and pairs beyond the
+    count are left untouched."""
+    n, w, h, D = 4, 176, 104, 64
+    lefts, rights = pairs(n, w, h, D - 1, seed0=4400)
+    params = P.validate_params(P.Params(num_disparities=D, pyramid_levels=1,
+                                        cost_mode=P.CostMode.Census, p1=8, p2=32,
+                                        adaptive_p2=True))
+    want = singles(lefts, rights, params)
+    L = batch_tensor(lefts, torch.uint8)
+    R = batch_tensor(rights, torch.uint8)
+    bm = P.BatchMatcher(w, h, n, params)
+    for count in (2, 4, 1):
+        d16 = out_tensor(n, h, w, torch.int16, 1234)
+        bm.compute(L, R, d16, count=count)
+        torch.cuda.synchronize()
+        for i in range(n):
+            if i < count:
+                U.assert_same(f"count {count} pair {i}", d16[i].cpu().numpy(), want[i][0])
+            else:
+                assert bool((d16[i] == 1234).all()), f"count {count} wrote pair {i}"
+    g16 = out_tensor(n, h, w, torch.int16, 1234)
+    bm.capture(L, R, g16, count=3)
+    for rep in range(2):
+        g16.fill_(1234)
+        torch.cuda.synchronize()
+        bm.launch()
+        torch.cuda.synchronize()
+        for i in range(3):
+            U.assert_same(f"graph rep {rep} pair {i}", g16[i].cpu().numpy(), want[i][0])
+        assert bool((g16[3] == 1234).all())
+    bm.close()

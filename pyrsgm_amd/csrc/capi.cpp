@@ -139,6 +139,14 @@ int exactPathsNeeded(const LevelCall& c)
     return cmax + 2 * p2max >= 32767 ? 1 : 0;
 }
 
+int narrowVolumes(const LevelCall& c)
+{
+    if (c.cost != 1 || c.Dp > 64) return 0;
+    long p2max = c.P2;
+    for (int i = 0; i < 256; ++i) p2max = std::max<long>(p2max, c.lut[i]);
+    return 24 + 2 * p2max < 255 ? 1 : 0;
+}
+
 hipError_t launchPaths(const PathArgs& a, int mode, int adaptive, hipStream_t s)
 {
     if (mode == kRefine) return launchPathsRefine(a, adaptive, s);
@@ -178,6 +186,7 @@ hipError_t launchLevel(const LevelCall& c, Workspace& ws, hipStream_t s)
     a.endX = c.W + std::min(c.minD, 0);
     a.numPaths = c.numPaths;
     a.exact = exactPathsNeeded(c);
+    a.narrow = c.narrow;
     a.disp = ws.disp;
     a.conf = ws.conf;
     a.keys = ws.keys;
@@ -275,6 +284,7 @@ int computeSingleScale(
     c.numPaths = std::min(3, std::max(1, numPaths));
     c.mode = kSingle;
     c.adaptive = buildP2Lut(P2, adaptiveP2, p2Alpha, p2Beta, p2LowerMult, p2UpperMult, c.lut);
+    c.narrow = narrowVolumes(c);
     c.batch = b;
     c.batch.ws = ws->slice;
     const hipError_t e = launchLevel(c, *ws, stream);
@@ -344,6 +354,7 @@ int computeLevel(
     c.mode = refineLevel ? kRefine : kCoarse;
     c.minConf = fminf(fmaxf(minConfidence, 0.0f), 1.0f);
     c.adaptive = buildP2Lut(P2, adaptiveP2, p2Alpha, p2Beta, p2LowerMult, p2UpperMult, c.lut);
+    c.narrow = narrowVolumes(c);
     c.batch = b;
     c.batch.ws = ws->slice;
     const hipError_t e = launchLevel(c, *ws, stream);

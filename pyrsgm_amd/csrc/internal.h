@@ -74,9 +74,17 @@ struct LevelCall {
     int mode;        // Mode
     float minConf;
     int adaptive;
+    int narrow;      // byte volumes (narrowVolumes)
     Batch batch;
     int32_t lut[256];
 };
+
+// Byte volumes: census levels of at most 64 labels whose every cost and path
+// cost fits in a byte.  A path cost is at most its pixel's cost plus the
+// largest penalty (Cmax + max(P2') with Cmax = 24), and the fast path step is
+// exact below a ceiling M when Cmax + 2 * max(P2') < M; with M = 255 the
+// volumes store 255 - C and 255 - L (0: SHRT_MAX) and every value is unchanged.
+int narrowVolumes(const LevelCall& c);
 
 hipError_t launchLevel(const LevelCall& c, Workspace& ws, hipStream_t stream);
 
@@ -107,6 +115,7 @@ struct PathArgs {
     int startX, endX;          // single-scale L->R column range
     int numPaths;
     int exact;                 // pyramid: Cmax + 2*max(P2') may reach SHRT_MAX
+    int narrow;                // byte volumes (see narrowVolumes)
     int nLRBlocks;             // combined launch: blocks [0, nLR) run L->R,
     int nRLBlocks;             // [nLR, nLR+nRL) R->L, the rest T->B
     int16_t* disp;

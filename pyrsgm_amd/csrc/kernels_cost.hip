@@ -155,6 +155,22 @@ __device__ __forceinline__ void storeCost8(int16_t* dst, const int (&v)[8])
     store8(dst, e);
 }
 
+// Byte form (census levels whose costs and path costs fit a byte): 255 - C,
+// 0 for SHRT_MAX; element index `at` addresses bytes.
+template <bool NARROW>
+__device__ __forceinline__ void storeCensus8(int16_t* C, size_t at, const int (&v)[8])
+{
+    if constexpr (NARROW) {
+        uint32_t w[2] = {0u, 0u};
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            w[i >> 2] |= (uint32_t)(v[i] >= kMax ? 0 : 255 - v[i]) << (8 * (i & 3));
+        *reinterpret_cast<uint2*>(reinterpret_cast<uint8_t*>(C) + at) = make_uint2(w[0], w[1]);
+    } else {
+        storeCost8(C + at, v);
+    }
+}
+
 // ---- Census cost over the full label range (sgm_kernels.cu:303-327) --------
 // Block: 256 threads = NPX pixels x (Dp/8) label octets of one row segment; the
 // right-view census words the segment can reach are staged in LDS.
@@ -162,6 +178,7 @@ __device__ __forceinline__ void storeCost8(int16_t* dst, const int (&v)[8])
 // [minD, min(maxD, x)] or whose right column is outside the image is stored as
 // SHRT_MAX: that is exactly the reference's labelValid = geometry && C < MAX
 // (sgm_kernels.cu:1135-1143), so the path kernels test "C < SHRT_MAX" alone.
+template <bool NARROW>
 __global__ __launch_bounds__(256) void kCensusCost(const uint32_t* cen, int16_t* C, int W, int H,
                                                    int D, int Dp, int minD, int maxD, int geomMask,
                                                    size_t wsStride)
@@ -197,7 +214,7 @@ __global__ __launch_bounds__(256) void kCensusCost(const uint32_t* cen, int16_t*
         v[i] = d >= D ? kMax : (in ? __popc(cl ^ s_r[x2 - base2]) : 24);
         if (geomMask && (!in || minD + d > hiReal)) v[i] = kMax;
     }
-    storeCost8(C + ((size_t)y * W + x) * Dp + d0, v);
+    storeCensus8<NARROW>(C, ((size_t)y * W + x) * Dp + d0, v);
 }
 
 // ---- refinement labels (sgm_kernels.cu:955-1000) ----------------------------
@@ -259,7 +276,7 @@ __device__ __forceinline__ int32_t labelWord(int base, int valid, uint8_t im)
 // global memory.  Every global load of a thread (staged words, parent
 // disparity and confidence, left census word, intensity) is issued before the
 // first wait.
-template <bool STAGED>
+template <bool STAGED, bool NARROW>
 __global__ __launch_bounds__(256) void kCensusLocal(const uint32_t* cen, RefineIn r, int32_t* labels,
                                                     const uint8_t* img, int istep, int16_t* C,
                                                     int W, int H, int D, int Dp, int minD, int maxD,
@@ -327,7 +344,7 @@ __global__ __launch_bounds__(256) void kCensusLocal(const uint32_t* cen, RefineI
         const uint32_t cr = STAGED ? s_r[x2 - xb] : R[x2];
         v[i] = ok ? __popc(cl ^ cr) : kMax;
     }
-    storeCost8(C + p * Dp + d0, v);
+    storeCensus8<NARROW>(C, p * Dp + d0, v);
 }
 
 // ---- BT cost + horizontal window sum (sgm_kernels.cu:139-239, :354-379) ------
@@ -561,13 +578,23 @@ hipError_t launchCost(const LevelCall& c, Workspace& ws, hipStream_t s)
         dim3 grid((c.W + npx - 1) / npx, c.H, c.batch.n);
         const uint32_t* cen = reinterpret_cast<const uint32_t*>(ws.desc);
         const long span = (long)npx + c.maxD - c.minD;  // staged right columns
-        if (c.cost == 1 && span <= kMaxStaged)
-            kCensusLocal<true><<<grid, 256, (size_t)span * 4, s>>>(cen, r, ws.labels, c.left,
-                                                                   c.lstep, ws.C, c.W, c.H, c.D,
-                                                                   c.Dp, c.minD, c.maxD, c.batch);
+        const size_t lds = (size_t)span * 4;
+        if (c.cost == 1 && span <= kMaxStaged && c.narrow)
+            kCensusLocal<true, true><<<grid, 256, lds, s>>>(cen, r, ws.labels, c.left, c.lstep,
+                                                            ws.C, c.W, c.H, c.D, c.Dp, c.minD,
+                                                            c.maxD, c.batch);
+        else if (c.cost == 1 && span <= kMaxStaged)
+            kCensusLocal<true, false><<<grid, 256, lds, s>>>(cen, r, ws.labels, c.left, c.lstep,
+                                                             ws.C, c.W, c.H, c.D, c.Dp, c.minD,
+                                                             c.maxD, c.batch);
+        else if (c.cost == 1 && c.narrow)
+            kCensusLocal<false, true><<<grid, 256, 0, s>>>(cen, r, ws.labels, c.left, c.lstep,
+                                                           ws.C, c.W, c.H, c.D, c.Dp, c.minD,
+                                                           c.maxD, c.batch);
         else if (c.cost == 1)
-            kCensusLocal<false><<<grid, 256, 0, s>>>(cen, r, ws.labels, c.left, c.lstep, ws.C, c.W,
-                                                     c.H, c.D, c.Dp, c.minD, c.maxD, c.batch);
+            kCensusLocal<false, false><<<grid, 256, 0, s>>>(cen, r, ws.labels, c.left, c.lstep,
+                                                            ws.C, c.W, c.H, c.D, c.Dp, c.minD,
+                                                            c.maxD, c.batch);
         else
             kBTLocal<<<grid, 256, 0, s>>>(ws.desc, r, ws.labels, c.left, c.lstep, ws.C, c.W, c.H,
                                           c.D, c.Dp, c.minD, c.maxD, half, c.batch);
@@ -576,9 +603,13 @@ hipError_t launchCost(const LevelCall& c, Workspace& ws, hipStream_t s)
     if (c.cost == 1) {
         dim3 grid((c.W + npx - 1) / npx, c.H, c.batch.n);
         const size_t lds = (size_t)(npx + c.Dp - 1) * sizeof(uint32_t);
-        kCensusCost<<<grid, 256, lds, s>>>(reinterpret_cast<const uint32_t*>(ws.desc), ws.C, c.W,
-                                           c.H, c.D, c.Dp, c.minD, c.maxD, c.mode == kCoarse,
-                                           c.batch.ws);
+        const uint32_t* cen = reinterpret_cast<const uint32_t*>(ws.desc);
+        if (c.narrow)
+            kCensusCost<true><<<grid, 256, lds, s>>>(cen, ws.C, c.W, c.H, c.D, c.Dp, c.minD,
+                                                     c.maxD, c.mode == kCoarse, c.batch.ws);
+        else
+            kCensusCost<false><<<grid, 256, lds, s>>>(cen, ws.C, c.W, c.H, c.D, c.Dp, c.minD,
+                                                      c.maxD, c.mode == kCoarse, c.batch.ws);
         return hipGetLastError();
     }
     const int dc = c.Dp < kChunkD ? c.Dp : kChunkD;

@@ -1,5 +1,8 @@
 // kernels_wta.hip -- path sums + winner-take-all for one level.
 //
+// Volumes are int16 (SHRT_MAX - L) or, in the byte form, uint8 (255 - L, 0
+// for SHRT_MAX); both are converted to L on load, so the rest is shared.
+//
 // Q = Dp/16 lanes per pixel, 16 labels per lane, pixels consecutive across
 // lanes, so every volume read is a run of full 16-byte loads.  Per pixel:
 //   * sum the three path costs in the reference's order and saturation:
@@ -36,7 +39,28 @@ __device__ __forceinline__ void load16(int (&v)[16], const int16_t* p)
     }
 }
 
-template <int Q, int MODE>
+// Byte volumes hold 255 - L, 0 standing for SHRT_MAX; returns the 16 path costs L.
+__device__ __forceinline__ void load16n(int (&v)[16], const uint8_t* p)
+{
+    const uint4 a = *reinterpret_cast<const uint4*>(p);
+    const uint32_t w[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int b = (int)((w[i >> 2] >> (8 * (i & 3))) & 0xFFu);
+        v[i] = b ? 255 - b : kMax;
+    }
+}
+
+template <bool NARROW>
+__device__ __forceinline__ void loadVol(int (&v)[16], const int16_t* vol, size_t off)
+{
+    if constexpr (NARROW)
+        load16n(v, reinterpret_cast<const uint8_t*>(vol) + off);
+    else
+        load16(v, vol + off);
+}
+
+template <int Q, int MODE, bool NARROW>
 __global__ __launch_bounds__(256) void kWTA(PathArgs a)
 {
     constexpr bool PYR = MODE != kSingle;
@@ -62,9 +86,9 @@ __global__ __launch_bounds__(256) void kWTA(PathArgs a)
     int my[16];
     {
         int rl[16], lr[16], tb[16];
-        load16(rl, Arl + off);
-        if (np >= 2) load16(lr, Alr + off);
-        if (np >= 3) load16(tb, Atb + off);
+        loadVol<NARROW>(rl, Arl, off);
+        if (np >= 2) loadVol<NARROW>(lr, Alr, off);
+        if (np >= 3) loadVol<NARROW>(tb, Atb, off);
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
             if constexpr (!PYR) {
@@ -169,32 +193,48 @@ __global__ __launch_bounds__(256) void kWTA(PathArgs a)
     confV[p] = cf;
 }
 
-template <int MODE>
+template <int MODE, bool NARROW>
 hipError_t dispatchWTA(const PathArgs& a, hipStream_t s)
 {
     const int Q = a.Dp / 16;
     const size_t threads = (size_t)a.W * a.H * Q;
     const unsigned blocks = (unsigned)((threads + 255) / 256);
+    const dim3 grid(blocks, a.batch.n);
+    if constexpr (NARROW) {  // at most 64 labels
+        switch (Q) {
+        case 1: kWTA<1, MODE, true><<<grid, 256, 0, s>>>(a); break;
+        case 2: kWTA<2, MODE, true><<<grid, 256, 0, s>>>(a); break;
+        case 4: kWTA<4, MODE, true><<<grid, 256, 0, s>>>(a); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     switch (Q) {
-    case 1: kWTA<1, MODE><<<dim3(blocks, a.batch.n), 256, 0, s>>>(a); break;
-    case 2: kWTA<2, MODE><<<dim3(blocks, a.batch.n), 256, 0, s>>>(a); break;
-    case 4: kWTA<4, MODE><<<dim3(blocks, a.batch.n), 256, 0, s>>>(a); break;
-    case 8: kWTA<8, MODE><<<dim3(blocks, a.batch.n), 256, 0, s>>>(a); break;
-    case 16: kWTA<16, MODE><<<dim3(blocks, a.batch.n), 256, 0, s>>>(a); break;
-    case 32: kWTA<32, MODE><<<dim3(blocks, a.batch.n), 256, 0, s>>>(a); break;
-    case 64: kWTA<64, MODE><<<dim3(blocks, a.batch.n), 256, 0, s>>>(a); break;
+    case 1: kWTA<1, MODE, false><<<grid, 256, 0, s>>>(a); break;
+    case 2: kWTA<2, MODE, false><<<grid, 256, 0, s>>>(a); break;
+    case 4: kWTA<4, MODE, false><<<grid, 256, 0, s>>>(a); break;
+    case 8: kWTA<8, MODE, false><<<grid, 256, 0, s>>>(a); break;
+    case 16: kWTA<16, MODE, false><<<grid, 256, 0, s>>>(a); break;
+    case 32: kWTA<32, MODE, false><<<grid, 256, 0, s>>>(a); break;
+    case 64: kWTA<64, MODE, false><<<grid, 256, 0, s>>>(a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+template <int MODE>
+hipError_t dispatchWTAMode(const PathArgs& a, hipStream_t s)
+{
+    return a.narrow ? dispatchWTA<MODE, true>(a, s) : dispatchWTA<MODE, false>(a, s);
 }
 
 } // namespace
 
 hipError_t launchWTA(const PathArgs& a, int mode, hipStream_t s)
 {
-    if (mode == kRefine) return dispatchWTA<kRefine>(a, s);
-    if (mode == kCoarse) return dispatchWTA<kCoarse>(a, s);
-    return dispatchWTA<kSingle>(a, s);
+    if (mode == kRefine) return dispatchWTAMode<kRefine>(a, s);
+    if (mode == kCoarse) return dispatchWTAMode<kCoarse>(a, s);
+    return dispatchWTAMode<kSingle>(a, s);
 }
 
 } // namespace pyr

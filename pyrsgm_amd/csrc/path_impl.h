@@ -111,12 +111,16 @@ __device__ __forceinline__ void bstore(const int (&v)[K], __amdgpu_buffer_rsrc_t
     }
 }
 
-// K labels of one lane into the LDS tile (int16 each).
-template <int K>
+// K labels of one lane into the LDS tile (int16 each, or one byte each in
+// the byte-volume form, which has K = 1).
+template <int K, bool NARROW = false>
 __device__ __forceinline__ void tstore(const int (&v)[K], char* p)
 {
     auto pk = [&](int i) { return (uint32_t)(uint16_t)v[i] | ((uint32_t)v[i + 1] << 16); };
-    if constexpr (K == 1) {
+    if constexpr (NARROW) {
+        static_assert(K == 1, "byte volumes have one label per lane");
+        *reinterpret_cast<uint8_t*>(p) = (uint8_t)v[0];
+    } else if constexpr (K == 1) {
         *reinterpret_cast<uint16_t*>(p) = (uint16_t)v[0];
     } else if constexpr (K == 2) {
         *reinterpret_cast<uint32_t*>(p) = pk(0);
@@ -297,9 +301,10 @@ struct ChunkAddr {
 // finished path costs back.  Each chunk of U steps ends with one workgroup
 // barrier.  LDS plan:
 //  * costs: 3 slots (chunks k, k+1 in use, k+2 being filled); in a slot each
-//    compute lane owns 16 dwords (its K labels for the U steps, a dword per
-//    label pair, or per label when K = 1), padded to 80 bytes so that the
-//    lane's four 16-byte reads hit disjoint banks;
+//    compute lane owns its K labels for the U steps: 16 dwords (a dword per
+//    label pair) when K > 1, 16 halfwords when K = 1, padded by 16 bytes so
+//    that the lanes' 16-byte reads hit disjoint banks.  The 16-bit form keeps
+//    the K = 1 workgroup at 15 KB, so that ten fit in a CU's LDS;
 //  * per-step words: 3 slots of 64 dwords, [line][step slot];
 //  * path costs: 2 tiles (chunk k being written, chunk k-1 being stored).
 //    Horizontal paths [line][step slot][label], each line's row padded by one
@@ -307,15 +312,30 @@ struct ChunkAddr {
 //    step); vertical path [step][line][label].  A horizontal line's row and a
 //    vertical step's row are each one contiguous run of the path-cost volume,
 //    written with 16-byte stores.
-template <int G, int K>
+template <int G, int K, bool NARROW = false>
 struct Lds {
+    static_assert(!NARROW || K == 1, "byte volumes have one label per lane");
     static constexpr int U = prefetchSteps<K>();
     static constexpr int LPW = 64 / G;
-    static constexpr int VB = G * K * 2;               // bytes of one pixel's labels
+    static constexpr int ESZ = NARROW ? 1 : 2;         // bytes per label in volumes and tiles
+    static constexpr int VB = G * K * ESZ;             // bytes of one pixel's labels
     static constexpr int W32 = (K + 1) / 2;            // cost dwords per lane and step
-    static_assert(U * W32 == 16, "a compute lane's chunk of costs is 16 dwords");
-    static constexpr int LANE = 80;                    // bytes per compute lane in a cost slot
-    static constexpr int CSLOT = 64 * LANE;
+    // A lane's U steps of costs: bytes (byte volumes, 16 bytes), 16-bit costs
+    // (K = 1, 32 bytes) or W32 dwords per step (64 bytes); read as CQ 16-byte
+    // pieces.
+    static constexpr bool PACK16 = K == 1 && !NARROW;
+    static constexpr int CQ = NARROW ? 1 : (PACK16 ? 2 : 4);
+    static_assert(U * (NARROW ? 1 : (PACK16 ? 2 : 4 * W32)) == CQ * 16,
+                  "a compute lane's chunk of costs is CQ pieces");
+    // Bytes per compute lane in a cost slot: 16 bytes of padding keep the
+    // lanes' 16-byte reads on distinct banks.  Byte slots are unpadded (16-byte
+    // reads at a 16-byte stride are conflict-free) and each line's 16 lanes are
+    // followed by LPAD bytes, which spreads the memory wave's byte writes of
+    // different lines over different banks.
+    static constexpr int LANE = NARROW ? 16 : CQ * 16 + 16;
+    static constexpr int LPAD = NARROW ? 16 : 0;
+    static constexpr int CSLOT = 64 * LANE + LPW * LPAD;
+    __device__ static constexpr int laneOff(int l) { return l * LANE + (l / G) * LPAD; }
     static constexpr int WSLOT = 64 * 4;
     static constexpr int ROWH = U * VB + VB;           // horizontal tile row, bytes
     static constexpr int TSLOT = (LPW * ROWH > U * LPW * VB) ? LPW * ROWH : U * LPW * VB;
@@ -347,7 +367,7 @@ __device__ __forceinline__ void waveLdsFence()
 }
 
 // MODE: kSingle, kCoarse or kRefine.
-template <int G, int K, int MODE, bool ADAPT, bool EXACT, int DIR>
+template <int G, int K, int MODE, bool ADAPT, bool EXACT, bool NARROW, int DIR>
 __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int lineGroup,
                                         const int* s_lut, int* s_xchg, char* s_lds)
 {
@@ -360,8 +380,9 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
     constexpr bool PYR = MODE != kSingle;
     constexpr bool REFINE = MODE == kRefine;
     constexpr bool WORDS = REFINE || ADAPT;  // per-step words are loaded
-    constexpr int VB = GK * 2;  // bytes of one pixel's label vector
-    using TL = Lds<G, K>;
+    using TL = Lds<G, K, NARROW>;
+    constexpr int ESZ = TL::ESZ;
+    constexpr int VB = GK * ESZ;  // bytes of one pixel's label vector
     constexpr int W32 = TL::W32;
 
     const bool memWave = threadIdx.x >= 64;
@@ -372,8 +393,6 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
     const int W = a.W, H = a.H, D = a.D;
     const int nLines = VERT ? W : H;
     const int line0 = lineGroup * LPW;
-    const int line = line0 + j;
-    const bool lineOk = line < nLines;
     const int nValid = imin(LPW, nLines - line0);
     int s0 = 0, s1 = VERT ? H : W;
     if constexpr (!PYR && DIR == kLR) {
@@ -480,14 +499,19 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
             if (row < nRows && col < nCols) {
                 const int jj = VERT ? col : row;
                 const int u = VERT ? row : (RL ? nSteps - 1 - col : col);
-                const int d0 = (ob - col * VB) / 2;
-                char* const p = slot + (jj * G + d0 / K) * TL::LANE;
+                const int d0 = (ob - col * VB) / ESZ;
+                char* const p = slot + TL::laneOff(jj * G + d0 / K);
                 const u4v q = m.q[i];
-                if constexpr (K == 1) {
+                if constexpr (NARROW) {
+#pragma unroll
+                    for (int t = 0; t < 16; ++t)
+                        *reinterpret_cast<uint8_t*>(p + t * TL::LANE + u) =
+                            (uint8_t)(q[t >> 2] >> (8 * (t & 3)));
+                } else if constexpr (K == 1) {
 #pragma unroll
                     for (int t = 0; t < 8; ++t)
-                        *reinterpret_cast<uint32_t*>(p + t * TL::LANE + u * 4) =
-                            (t & 1) ? (q[t >> 1] >> 16) : (q[t >> 1] & 0xFFFFu);
+                        *reinterpret_cast<uint16_t*>(p + t * TL::LANE + u * 2) =
+                            (uint16_t)((t & 1) ? (q[t >> 1] >> 16) : q[t >> 1]);
                 } else if constexpr (K == 2) {
 #pragma unroll
                     for (int t = 0; t < 4; ++t) *reinterpret_cast<uint32_t*>(p + t * TL::LANE + u * 4) = q[t];
@@ -696,19 +720,19 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
             for (int k = 0; k < K; ++k) Hs[k] = active ? Lc[k] : 0;
             minPrev = active ? minCur : 0;
         }
-        tstore<K>(Hc, tp);
+        tstore<K, NARROW>(Hc, tp);
     };
 
     // ---- compute wave --------------------------------------------------------
     struct CBuf {
-        u4v q[4];
+        u4v q[TL::CQ];
         int w;
     };
     // This lane's costs of chunk k (and its row's per-step word) from LDS.
     auto cread = [&](CBuf& c, int k) {
-        const char* const p = s_lds + (k % 3) * TL::CSLOT + lane * TL::LANE;
+        const char* const p = s_lds + (k % 3) * TL::CSLOT + TL::laneOff(lane);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) c.q[i] = *reinterpret_cast<const u4v*>(p + 16 * i);
+        for (int i = 0; i < TL::CQ; ++i) c.q[i] = *reinterpret_cast<const u4v*>(p + 16 * i);
         if constexpr (WORDS)
             c.w = *reinterpret_cast<const int*>(s_lds + TL::OFF_W + (k % 3) * TL::WSLOT +
                                                 (j * 16 + r16) * 4);
@@ -747,15 +771,22 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
                 const int p2 = ADAPT ? rowBcast<u>(p2v) : a.P2;
                 const bool moved = REFINE && ((moveMask >> u) & 1u);
                 uint32_t cw[W32];
+                if constexpr (NARROW) {
+                    cw[0] = (c.q[0][u >> 2] >> (8 * (u & 3))) & 0xFFu;
+                } else if constexpr (TL::PACK16) {
+                    const uint32_t x = c.q[u >> 3][(u >> 1) & 3];
+                    cw[0] = (u & 1) ? (x >> 16) : (x & 0xFFFFu);
+                } else {
 #pragma unroll
-                for (int w = 0; w < W32; ++w) cw[w] = c.q[(u * W32 + w) >> 2][(u * W32 + w) & 3];
+                    for (int w = 0; w < W32; ++w) cw[w] = c.q[(u * W32 + w) >> 2][(u * W32 + w) & 3];
+                }
                 // Tile slot of step u: horizontal rows run in address order (R->L
                 // reversed), the vertical tile is step-major.
                 int tb;
                 if constexpr (VERT)
-                    tb = (u * LPW + j) * VB + g * K * 2;
+                    tb = (u * LPW + j) * VB + g * K * ESZ;
                 else
-                    tb = j * TL::ROWH + (RL ? (full ? U - 1 - u : nSteps - 1 - u) : u) * VB + g * K * 2;
+                    tb = j * TL::ROWH + (RL ? (full ? U - 1 - u : nSteps - 1 - u) : u) * VB + g * K * ESZ;
                 step(uc, cw, p2, moved, bigMask, shv, tile + tb);
             }
         };
@@ -784,12 +815,12 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
     }
 }
 
-template <int G, int K, int MODE, bool ADAPT, bool EXACT>
+template <int G, int K, int MODE, bool ADAPT, bool EXACT, bool NARROW>
 __global__ __launch_bounds__(128) void kPaths(PathArgs a)
 {
     __shared__ int s_lut[256];
     __shared__ int s_xchg[MODE == kRefine ? 64 * K : 1];
-    __shared__ __attribute__((aligned(16))) char s_lds[Lds<G, K>::BYTES];
+    __shared__ __attribute__((aligned(16))) char s_lds[Lds<G, K, NARROW>::BYTES];
     if constexpr (ADAPT) {
         for (int i = threadIdx.x; i < 256; i += 128) s_lut[i] = a.lut[i];
         __syncthreads();
@@ -797,16 +828,17 @@ __global__ __launch_bounds__(128) void kPaths(PathArgs a)
     const int b = blockIdx.x;
     const size_t pr = blockIdx.y;  // pair
     if (b < a.nLRBlocks)
-        runLine<G, K, MODE, ADAPT, EXACT, kLR>(a, pr,  b, s_lut, s_xchg, s_lds);
+        runLine<G, K, MODE, ADAPT, EXACT, NARROW, kLR>(a, pr, b, s_lut, s_xchg, s_lds);
     else if (b < a.nLRBlocks + a.nRLBlocks)
-        runLine<G, K, MODE, ADAPT, EXACT, kRL>(a, pr,  b - a.nLRBlocks, s_lut, s_xchg, s_lds);
+        runLine<G, K, MODE, ADAPT, EXACT, NARROW, kRL>(a, pr, b - a.nLRBlocks, s_lut, s_xchg, s_lds);
     else
-        runLine<G, K, MODE, ADAPT, EXACT, kTB>(a, pr,  b - a.nLRBlocks - a.nRLBlocks, s_lut, s_xchg, s_lds);
+        runLine<G, K, MODE, ADAPT, EXACT, NARROW, kTB>(a, pr, b - a.nLRBlocks - a.nRLBlocks, s_lut,
+                                                       s_xchg, s_lds);
 }
 
 inline int linesPerWave(int Dp) { return Dp <= 16 ? 4 : (Dp <= 32 ? 2 : 1); }
 
-template <int MODE, bool ADAPT, bool EXACT>
+template <int MODE, bool ADAPT, bool EXACT, bool NARROW>
 hipError_t dispatchPaths(const PathArgs& a, hipStream_t s)
 {
     const int lpw = linesPerWave(a.Dp);
@@ -815,14 +847,24 @@ hipError_t dispatchPaths(const PathArgs& a, hipStream_t s)
     b.nRLBlocks = (a.H + lpw - 1) / lpw;
     const int nTB = a.numPaths >= 3 ? (a.W + lpw - 1) / lpw : 0;
     const int blocks = b.nLRBlocks + b.nRLBlocks + nTB;
+    const dim3 grid(blocks, a.batch.n);
+    if constexpr (NARROW) {
+        switch (a.Dp) {
+        case 16: kPaths<16, 1, MODE, ADAPT, EXACT, true><<<grid, 128, 0, s>>>(b); break;
+        case 32: kPaths<32, 1, MODE, ADAPT, EXACT, true><<<grid, 128, 0, s>>>(b); break;
+        case 64: kPaths<64, 1, MODE, ADAPT, EXACT, true><<<grid, 128, 0, s>>>(b); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     switch (a.Dp) {
-    case 16: kPaths<16, 1, MODE, ADAPT, EXACT><<<dim3(blocks, a.batch.n), 128, 0, s>>>(b); break;
-    case 32: kPaths<32, 1, MODE, ADAPT, EXACT><<<dim3(blocks, a.batch.n), 128, 0, s>>>(b); break;
-    case 64: kPaths<64, 1, MODE, ADAPT, EXACT><<<dim3(blocks, a.batch.n), 128, 0, s>>>(b); break;
-    case 128: kPaths<64, 2, MODE, ADAPT, EXACT><<<dim3(blocks, a.batch.n), 128, 0, s>>>(b); break;
-    case 256: kPaths<64, 4, MODE, ADAPT, EXACT><<<dim3(blocks, a.batch.n), 128, 0, s>>>(b); break;
-    case 512: kPaths<64, 8, MODE, ADAPT, EXACT><<<dim3(blocks, a.batch.n), 128, 0, s>>>(b); break;
-    case 1024: kPaths<64, 16, MODE, ADAPT, EXACT><<<dim3(blocks, a.batch.n), 128, 0, s>>>(b); break;
+    case 16: kPaths<16, 1, MODE, ADAPT, EXACT, false><<<grid, 128, 0, s>>>(b); break;
+    case 32: kPaths<32, 1, MODE, ADAPT, EXACT, false><<<grid, 128, 0, s>>>(b); break;
+    case 64: kPaths<64, 1, MODE, ADAPT, EXACT, false><<<grid, 128, 0, s>>>(b); break;
+    case 128: kPaths<64, 2, MODE, ADAPT, EXACT, false><<<grid, 128, 0, s>>>(b); break;
+    case 256: kPaths<64, 4, MODE, ADAPT, EXACT, false><<<grid, 128, 0, s>>>(b); break;
+    case 512: kPaths<64, 8, MODE, ADAPT, EXACT, false><<<grid, 128, 0, s>>>(b); break;
+    case 1024: kPaths<64, 16, MODE, ADAPT, EXACT, false><<<grid, 128, 0, s>>>(b); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -832,8 +874,13 @@ template <int MODE>
 hipError_t launchPathsMode(const PathArgs& a, int adaptive, hipStream_t s)
 {
     if (MODE != kSingle && a.exact)
-        return adaptive ? dispatchPaths<MODE, true, true>(a, s) : dispatchPaths<MODE, false, true>(a, s);
-    return adaptive ? dispatchPaths<MODE, true, false>(a, s) : dispatchPaths<MODE, false, false>(a, s);
+        return adaptive ? dispatchPaths<MODE, true, true, false>(a, s)
+                        : dispatchPaths<MODE, false, true, false>(a, s);
+    if (a.narrow)
+        return adaptive ? dispatchPaths<MODE, true, false, true>(a, s)
+                        : dispatchPaths<MODE, false, false, true>(a, s);
+    return adaptive ? dispatchPaths<MODE, true, false, false>(a, s)
+                    : dispatchPaths<MODE, false, false, false>(a, s);
 }
 
 } // namespace path

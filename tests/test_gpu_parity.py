@@ -254,3 +254,32 @@ def test_census_16bit_volumes(ws, kind):
     U.assert_same("disp16", d16, want.disp16)
     U.assert_same("disp32", d32, want.disp32)
     U.assert_same("conf", cf, want.conf)
+
+
+@pytest.mark.parametrize("P2", [115, 116])
+@pytest.mark.parametrize("kind", ["single", "coarse", "refine"])
+def test_census_byte_volume_limit(ws, kind, P2):
+    """Fixed P2 at the byte-volume limit: 24 + 2 * 115 < 255 runs on byte
+    volumes with path costs up to 139; 24 + 2 * 116 does not and stays 16-bit.
+    Both equal the oracle."""
+    w, h = 176, 80
+    left, right = pair(w, h, 60, seed=93 + P2)
+    base = dict(P1=10, P2=P2, cost=1, adaptive=0)
+    if kind == "single":
+        kw = dict(block=5, post=3, paths=3, **base)
+        rc, d16, d32, cf = U.single_scale(ws, left, right, 64, **kw)
+        want = O.compute_single_scale(left, right, 64, **kw)
+    elif kind == "coarse":
+        kw = dict(block=3, post=1, paths=3, **base)
+        rc, d16, d32, cf = U.level(ws, left, right, 64, 63, **kw)
+        want = O.compute_level(left, right, 64, 63, **kw)
+    else:
+        pl, pr = O.pyrdown(left), O.pyrdown(right)
+        parent = O.compute_level(pl, pr, 32, 31, block=3, post=1, want32=False, **base)
+        kw = dict(block=3, post=3, paths=3, refine=True, min_conf=0.05, **base)
+        rc, d16, d32, cf = U.level(ws, left, right, 16, 63, parent.disp16, parent.conf, **kw)
+        want = O.compute_level(left, right, 16, 63, parent.disp16, parent.conf, **kw)
+    assert rc == 0 and want.rc == 0
+    U.assert_same("disp16", d16, want.disp16)
+    U.assert_same("disp32", d32, want.disp32)
+    U.assert_same("conf", cf, want.conf)

@@ -39,6 +39,7 @@ __device__ __forceinline__ int dpp(int old, int src)
 __device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
 __device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }
 __device__ __forceinline__ uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
+__device__ __forceinline__ uint32_t umax(uint32_t a, uint32_t b) { return a > b ? a : b; }
 __device__ __forceinline__ int sat16(int v) { return imin(32767, imax(-32768, v)); }
 
 // Value of the next-lower lane of the group; the group's first lane gets `fill`.

@@ -270,19 +270,22 @@ __device__ __forceinline__ int32_t labelWord(int base, int valid, uint8_t im)
 }
 
 // ---- Census cost at refinement levels (sgm_kernels.cu:1112-1133) ------------
-// Block: NPX pixels x (Dp/8) label octets of one row.  A label's right column
-// x - real lies in [x0 - maxD, x0 + NPX - 1 - minD]; when that span is modest
-// (STAGED) the right census words are staged in LDS first, otherwise read from
-// global memory.  Every global load of a thread (staged words, parent
-// disparity and confidence, left census word, intensity) is issued before the
-// first wait.
+// Block: NPX pixels x (Dp/8) label octets of kLocalRows rows.  A label's right
+// column x - real lies in [x0 - maxD, x0 + NPX - 1 - minD]; when that span is
+// modest (STAGED) the right census words of the block's rows are staged in LDS
+// first, otherwise read from global memory.  Every global load of a thread
+// (staged words, parent disparity and confidence, left census word,
+// intensity, for all its rows) is issued before the first wait.
+constexpr int kLocalRows = 4;
+
 template <bool STAGED, bool NARROW>
 __global__ __launch_bounds__(256) void kCensusLocal(const uint32_t* cen, RefineIn r, int32_t* labels,
                                                     const uint8_t* img, int istep, int16_t* C,
                                                     int W, int H, int D, int Dp, int minD, int maxD,
                                                     Batch b)
 {
-    constexpr int NPRE = 4;  // staged words per thread loaded ahead
+    constexpr int ROWS = kLocalRows;
+    constexpr int NPRE = 2;  // staged words per thread and row loaded ahead
     extern __shared__ uint32_t s_r[];
     const size_t pr = blockIdx.z;
     cen = pairAt(cen, pr, b.ws);
@@ -293,58 +296,77 @@ __global__ __launch_bounds__(256) void kCensusLocal(const uint32_t* cen, RefineI
     r.pconf = pairAt(r.pconf, pr, b.pconf);
     const int tpp = Dp >> 3;
     const int npx = 256 / tpp;
-    const int y = blockIdx.y;
+    const int y0 = blockIdx.y * ROWS;
     const int x0 = blockIdx.x * npx;
-    const uint32_t* R = cen + (size_t)W * H + (size_t)y * W;
     const int xb = x0 - maxD;  // staged column 0
     const int span = npx + maxD - minD;
-    uint32_t pre[NPRE];
-    if constexpr (STAGED) {
-#pragma unroll
-        for (int k = 0; k < NPRE; ++k) {
-            const int i = threadIdx.x + k * 256;
-            const int xx = xb + i;
-            pre[k] = (i < span && xx >= 0 && xx < W) ? R[xx] : 0u;
-        }
-    }
     const int px = threadIdx.x / tpp;
     const int d0 = (threadIdx.x - px * tpp) * 8;
     const int x = x0 + px;
     const int xc = imin(x, W - 1);
-    const size_t p = (size_t)y * W + xc;
-    const int raw = parentRaw(r, xc, y);
-    const float cf = parentConf(r, xc, y);
-    const uint32_t cl = cen[p];
-    const uint8_t im = img[(size_t)y * istep + xc];
+    uint32_t pre[ROWS][NPRE];
+    int raw[ROWS];
+    float cf[ROWS];
+    uint32_t cl[ROWS];
+    uint8_t im[ROWS];
+#pragma unroll
+    for (int rr = 0; rr < ROWS; ++rr) {
+        const int y = imin(y0 + rr, H - 1);
+        if constexpr (STAGED) {
+            const uint32_t* R = cen + (size_t)W * H + (size_t)y * W;
+#pragma unroll
+            for (int k = 0; k < NPRE; ++k) {
+                const int i = threadIdx.x + k * 256;
+                const int xx = xb + i;
+                pre[rr][k] = (i < span && xx >= 0 && xx < W) ? R[xx] : 0u;
+            }
+        }
+        raw[rr] = parentRaw(r, xc, y);
+        cf[rr] = parentConf(r, xc, y);
+        cl[rr] = cen[(size_t)y * W + xc];
+        im[rr] = img[(size_t)y * istep + xc];
+    }
     if constexpr (STAGED) {
 #pragma unroll
-        for (int k = 0; k < NPRE; ++k) {
-            const int i = threadIdx.x + k * 256;
-            if (i < span) s_r[i] = pre[k];
-        }
-        for (int i = threadIdx.x + NPRE * 256; i < span; i += 256) {
-            const int xx = xb + i;
-            s_r[i] = (xx >= 0 && xx < W) ? R[xx] : 0u;
+        for (int rr = 0; rr < ROWS; ++rr) {
+#pragma unroll
+            for (int k = 0; k < NPRE; ++k) {
+                const int i = threadIdx.x + k * 256;
+                if (i < span) s_r[rr * span + i] = pre[rr][k];
+            }
+            const uint32_t* R = cen + (size_t)W * H + (size_t)imin(y0 + rr, H - 1) * W;
+            for (int i = threadIdx.x + NPRE * 256; i < span; i += 256) {
+                const int xx = xb + i;
+                s_r[rr * span + i] = (xx >= 0 && xx < W) ? R[xx] : 0u;
+            }
         }
         __syncthreads();
     }
     if (x >= W) return;
-    int base, valid;
-    refineFrom(r, raw, cf, x, D, minD, maxD, base, valid);
-    if (d0 == 0) labels[p] = labelWord(base, valid, im);
     const int lo = imax(minD, x - W + 1), hi = imin(maxD, x);
-    int v[8];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int d = d0 + i;
-        const int real = base + d;
-        const bool ok = valid && d < D && real >= lo && real <= hi;
-        // ok => 0 <= x - real < W and x - real in the staged span.
-        const int x2 = ok ? x - real : (STAGED ? xb : 0);
-        const uint32_t cr = STAGED ? s_r[x2 - xb] : R[x2];
-        v[i] = ok ? __popc(cl ^ cr) : kMax;
+    for (int rr = 0; rr < ROWS; ++rr) {
+        const int y = y0 + rr;
+        if (y >= H) break;
+        int base, valid;
+        refineFrom(r, raw[rr], cf[rr], x, D, minD, maxD, base, valid);
+        const size_t p = (size_t)y * W + x;
+        if (d0 == 0) labels[p] = labelWord(base, valid, im[rr]);
+        const uint32_t* R = cen + (size_t)W * H + (size_t)y * W;
+        const uint32_t* S = s_r + rr * span;
+        int v[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int d = d0 + i;
+            const int real = base + d;
+            const bool ok = valid && d < D && real >= lo && real <= hi;
+            // ok => 0 <= x - real < W and x - real in the staged span.
+            const int x2 = ok ? x - real : (STAGED ? xb : 0);
+            const uint32_t cr = STAGED ? S[x2 - xb] : R[x2];
+            v[i] = ok ? __popc(cl[rr] ^ cr) : kMax;
+        }
+        storeCensus8<NARROW>(C, p * Dp + d0, v);
     }
-    storeCensus8<NARROW>(C, p * Dp + d0, v);
 }
 
 // ---- BT cost + horizontal window sum (sgm_kernels.cu:139-239, :354-379) ------
@@ -565,8 +587,8 @@ hipError_t launchPrep(const LevelCall& c, Workspace& ws, hipStream_t s)
     return hipGetLastError();
 }
 
-// Right census words a refinement block may stage in LDS (16 KB).
-constexpr long kMaxStaged = 4096;
+// Right census words per row a refinement block may stage in LDS (4 rows: 32 KB).
+constexpr long kMaxStaged = 2048;
 
 hipError_t launchCost(const LevelCall& c, Workspace& ws, hipStream_t s)
 {
@@ -576,9 +598,10 @@ hipError_t launchCost(const LevelCall& c, Workspace& ws, hipStream_t s)
     if (c.mode == kRefine) {
         RefineIn r{c.pdisp, c.pconf, c.pdpitch, c.pcpitch, c.PW, c.PH, c.minConf};
         dim3 grid((c.W + npx - 1) / npx, c.H, c.batch.n);
+        if (c.cost == 1) grid.y = (c.H + kLocalRows - 1) / kLocalRows;
         const uint32_t* cen = reinterpret_cast<const uint32_t*>(ws.desc);
-        const long span = (long)npx + c.maxD - c.minD;  // staged right columns
-        const size_t lds = (size_t)span * 4;
+        const long span = (long)npx + c.maxD - c.minD;  // staged right columns per row
+        const size_t lds = (size_t)span * 4 * kLocalRows;
         if (c.cost == 1 && span <= kMaxStaged && c.narrow)
             kCensusLocal<true, true><<<grid, 256, lds, s>>>(cen, r, ws.labels, c.left, c.lstep,
                                                             ws.C, c.W, c.H, c.D, c.Dp, c.minD,

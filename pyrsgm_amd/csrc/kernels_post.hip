@@ -17,9 +17,9 @@
 namespace pyr {
 namespace {
 
-// Block: 64 x 8 pixels, two rows per thread (row segments of 64 pixels give
+// Block: 64 x 16 pixels, four rows per thread (row segments of 64 pixels give
 // whole-line stores for every output).
-constexpr int kPostW = 64, kPostH = 8;
+constexpr int kPostW = 64, kPostH = 16, kPostR = kPostH / 4;
 
 struct PostArgs {
     const int16_t* disp;
@@ -120,15 +120,15 @@ __global__ __launch_bounds__(256) void kPost(PostArgs args)
     const int x = bx + lx;
     const int xc = imin(x, a.W - 1);
     // Own pixels: raw disparity and confidence.
-    int raw[2];
-    float cf0[2];
+    int raw[kPostR];
+    float cf0[kPostR];
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
+    for (int r = 0; r < kPostR; ++r) {
         const int yc = imin(by + ly0 + 4 * r, a.H - 1);
         raw[r] = a.disp[(size_t)yc * a.W + xc];
         cf0[r] = a.conf[(size_t)yc * a.W + xc];
     }
-    int chk[2];
+    int chk[kPostR];
     if (a.median) {
         // Checked disparities of the tile and its one-pixel ring (in-image cells only).
         int hd[NH];
@@ -155,18 +155,18 @@ __global__ __launch_bounds__(256) void kPost(PostArgs args)
         }
         __syncthreads();
 #pragma unroll
-        for (int r = 0; r < 2; ++r) chk[r] = s_d[ly0 + 4 * r + 1][lx + 1];
+        for (int r = 0; r < kPostR; ++r) chk[r] = s_d[ly0 + 4 * r + 1][lx + 1];
     } else {
-        LrProbe pb[2];
+        LrProbe pb[kPostR];
 #pragma unroll
-        for (int r = 0; r < 2; ++r)
+        for (int r = 0; r < kPostR; ++r)
             pb[r] = lrLoad(a, xc, imin(by + ly0 + 4 * r, a.H - 1), raw[r], invalid);
 #pragma unroll
-        for (int r = 0; r < 2; ++r) chk[r] = lrDecide(a, xc, raw[r], pb[r], invalid);
+        for (int r = 0; r < kPostR; ++r) chk[r] = lrDecide(a, xc, raw[r], pb[r], invalid);
     }
     if (x >= a.W) return;
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
+    for (int r = 0; r < kPostR; ++r) {
         const int ly = ly0 + 4 * r;
         const int y = by + ly;
         if (y >= a.H) break;

@@ -39,27 +39,6 @@ __device__ __forceinline__ void load16(int (&v)[16], const int16_t* p)
     }
 }
 
-// Byte volumes hold 255 - L, 0 standing for SHRT_MAX; returns the 16 path costs L.
-__device__ __forceinline__ void load16n(int (&v)[16], const uint8_t* p)
-{
-    const uint4 a = *reinterpret_cast<const uint4*>(p);
-    const uint32_t w[4] = {a.x, a.y, a.z, a.w};
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const int b = (int)((w[i >> 2] >> (8 * (i & 3))) & 0xFFu);
-        v[i] = b ? 255 - b : kMax;
-    }
-}
-
-template <bool NARROW>
-__device__ __forceinline__ void loadVol(int (&v)[16], const int16_t* vol, size_t off)
-{
-    if constexpr (NARROW)
-        load16n(v, reinterpret_cast<const uint8_t*>(vol) + off);
-    else
-        load16(v, vol + off);
-}
-
 template <int Q, int MODE, bool NARROW>
 __global__ __launch_bounds__(256) void kWTA(PathArgs a)
 {
@@ -83,49 +62,138 @@ __global__ __launch_bounds__(256) void kWTA(PathArgs a)
     const size_t off = pc * a.Dp + (size_t)sub * 16;
     const int np = a.numPaths;
 
-    int my[16];
-    {
-        int rl[16], lr[16], tb[16];
-        loadVol<NARROW>(rl, Arl, off);
-        if (np >= 2) loadVol<NARROW>(lr, Alr, off);
-        if (np >= 3) loadVol<NARROW>(tb, Atb, off);
+    const int d0 = sub * 16;
+    uint32_t kmin;
+    int minCost, bestD, alt, c0, c1, c2;
+    if constexpr (NARROW) {
+        // Byte volumes keep every path cost below 255 (narrowVolumes), so each
+        // sum above equals "SHRT_MAX if a used path is SHRT_MAX at the label,
+        // else the plain sum", and the label's sum is 255 * np - S with
+        // S = sum of the stored bytes.  S is formed for label pairs in packed
+        // 16-bit fields (Sv[k]: labels 2k, 2k+1); a field is 0 where a used
+        // path is SHRT_MAX.  The argmin over (cost, d) is then the maximum of
+        // the packed key S * 16 + (15 - label).
+        const uint4 vr = *reinterpret_cast<const uint4*>(reinterpret_cast<const uint8_t*>(Arl) + off);
+        const uint4 vl = np >= 2 ? *reinterpret_cast<const uint4*>(reinterpret_cast<const uint8_t*>(Alr) + off)
+                                 : make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
+        const uint4 vt = np >= 3 ? *reinterpret_cast<const uint4*>(reinterpret_cast<const uint8_t*>(Atb) + off)
+                                 : make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
+        const uint32_t wr[4] = {vr.x, vr.y, vr.z, vr.w};
+        const uint32_t wl[4] = {vl.x, vl.y, vl.z, vl.w};
+        const uint32_t wt[4] = {vt.x, vt.y, vt.z, vt.w};
+        // Unused paths read 0xFF bytes: they pass the validity test, and their
+        // 255 per label is taken off again by the 255 * 3 below.
+        uint32_t Sv[8];
+        const bool pad = d0 >= D;  // D is a multiple of 16: a lane is all labels or all padding
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            if constexpr (!PYR) {
-                int acc = np >= 2 ? lr[i] : 0;
-                if (np >= 3) acc = (int)(int16_t)imin(kMax, acc + tb[i]);
-                my[i] = (int)(int16_t)imin(kMax, acc + rl[i]);
-            } else {
-                int old = np >= 2 ? lr[i] : 0;
-                if (np >= 3) old = imax(lr[i], tb[i]) == kMax ? kMax : imin(kMax - 1, lr[i] + tb[i]);
-                my[i] = np >= 2 ? (imax(old, rl[i]) == kMax ? kMax : imin(kMax - 1, old + rl[i]))
-                                : rl[i];
+        for (int k = 0; k < 8; ++k) {
+            const uint32_t sel = (k & 1) ? 0x0C030C02u : 0x0C010C00u;  // bytes 2k, 2k+1 -> 16-bit fields
+            const uint32_t a = __builtin_amdgcn_perm(0u, wr[k >> 1], sel);
+            const uint32_t b = __builtin_amdgcn_perm(0u, wl[k >> 1], sel);
+            const uint32_t c = __builtin_amdgcn_perm(0u, wt[k >> 1], sel);
+            const uint32_t sum = a + b + c;
+            typedef uint16_t h2 __attribute__((ext_vector_type(2)));
+            const h2 mn = __builtin_elementwise_min(__builtin_elementwise_min(
+                                                        __builtin_bit_cast(h2, a), __builtin_bit_cast(h2, b)),
+                                                    __builtin_bit_cast(h2, c));
+            const h2 one = __builtin_elementwise_min(mn, (h2){1, 1});
+            Sv[k] = pad ? 0u : __builtin_bit_cast(uint32_t, (h2)(__builtin_bit_cast(h2, sum) * one));
+        }
+        const int tot = 255 * 3;  // S counts 255 for every unused path
+        auto costOf = [&](uint32_t sf) { return sf ? tot - (int)sf : kMax; };
+        uint32_t best = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            typedef uint16_t h2 __attribute__((ext_vector_type(2)));
+            const h2 sv = __builtin_bit_cast(h2, Sv[k]);
+            const h2 on = __builtin_elementwise_min(sv, (h2){1, 1});
+            const h2 key = (h2)(sv * (h2){16, 16}) + (h2)(on * (h2){(uint16_t)(15 - 2 * k), (uint16_t)(14 - 2 * k)});
+            const uint32_t kk = __builtin_bit_cast(uint32_t, key);
+            best = umax(best, umax(kk & 0xFFFFu, kk >> 16));
+        }
+        if (pad)
+            kmin = 0xFFFFFFFFu;
+        else if (best == 0)
+            kmin = ((uint32_t)kMax << 16) | (uint32_t)d0;
+        else
+            kmin = ((uint32_t)costOf(best >> 4) << 16) | (uint32_t)(d0 + 15 - (int)(best & 15u));
+        kmin = gumin<Q>(kmin);
+        minCost = (int)(int16_t)(kmin >> 16);
+        bestD = (int)(kmin & 0xFFFFu);
+        const int bl = bestD - d0;  // best label's index in this lane (may be outside [0, 16))
+        // Field f of the packed sums (kMax outside this lane).
+        auto field = [&](int f) {
+            if (f < 0 || f >= 16) return kMax;
+            uint32_t x = Sv[0];
+#pragma unroll
+            for (int k = 1; k < 8; ++k) x = (f >> 1) == k ? Sv[k] : x;
+            return costOf((f & 1) ? (x >> 16) : (x & 0xFFFFu));
+        };
+        c0 = field(bl - 1);
+        c2 = field(bl + 1);
+        // alt: best sum outside labels bl-1 .. bl+1.
+        const int sh = imin(imax(bl - 1, -3), 17);
+        const uint32_t X = (sh >= 0 ? (7u << sh) : (7u >> (-sh))) & 0xFFFFu;
+        uint32_t bestAlt = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const uint32_t m2 = (X >> (2 * k)) & 3u;
+            const uint32_t keep = ((m2 & 1u) ? 0u : 0xFFFFu) | ((m2 & 2u) ? 0u : 0xFFFF0000u);
+            const uint32_t v = Sv[k] & keep;
+            bestAlt = umax(bestAlt, umax(v & 0xFFFFu, v >> 16));
+        }
+        alt = gmin<Q>(costOf(bestAlt));
+        if constexpr (Q > 1) {
+            c0 = gmin<Q>(c0);
+            c2 = gmin<Q>(c2);
+        }
+        c1 = minCost;
+    } else {
+        int my[16];
+        {
+            int rl[16], lr[16], tb[16];
+            load16(rl, Arl + off);
+            if (np >= 2) load16(lr, Alr + off);
+            if (np >= 3) load16(tb, Atb + off);
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                if constexpr (!PYR) {
+                    int acc = np >= 2 ? lr[i] : 0;
+                    if (np >= 3) acc = (int)(int16_t)imin(kMax, acc + tb[i]);
+                    my[i] = (int)(int16_t)imin(kMax, acc + rl[i]);
+                } else {
+                    int old = np >= 2 ? lr[i] : 0;
+                    if (np >= 3) old = imax(lr[i], tb[i]) == kMax ? kMax : imin(kMax - 1, lr[i] + tb[i]);
+                    my[i] = np >= 2 ? (imax(old, rl[i]) == kMax ? kMax : imin(kMax - 1, old + rl[i]))
+                                    : rl[i];
+                }
             }
         }
-    }
-
-    const int d0 = sub * 16;
-    uint32_t kmin = 0xFFFFFFFFu;
+        kmin = 0xFFFFFFFFu;
 #pragma unroll
-    for (int i = 0; i < 16; ++i)
-        if (d0 + i < D) kmin = umin(kmin, ((uint32_t)(uint16_t)my[i] << 16) | (uint32_t)(d0 + i));
-    kmin = gumin<Q>(kmin);
-    const int minCost = (int)(int16_t)(kmin >> 16);
-    const int bestD = (int)(kmin & 0xFFFFu);
-    int alt = kMax, c0 = kMax, c1 = kMax, c2 = kMax;
+        for (int i = 0; i < 16; ++i)
+            if (d0 + i < D) kmin = umin(kmin, ((uint32_t)(uint16_t)my[i] << 16) | (uint32_t)(d0 + i));
+        kmin = gumin<Q>(kmin);
+        minCost = (int)(int16_t)(kmin >> 16);
+        bestD = (int)(kmin & 0xFFFFu);
+        alt = kMax;
+        c0 = kMax;
+        c1 = kMax;
+        c2 = kMax;
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const int d = d0 + i;
-        if (d < D && abs(d - bestD) > 1) alt = imin(alt, my[i]);
-        if (d == bestD - 1) c0 = my[i];
-        if (d == bestD) c1 = my[i];
-        if (d == bestD + 1) c2 = my[i];
-    }
-    alt = gmin<Q>(alt);
-    if constexpr (Q > 1) {
-        c0 = gmin<Q>(c0);
-        c1 = gmin<Q>(c1);
-        c2 = gmin<Q>(c2);
+        for (int i = 0; i < 16; ++i) {
+            const int d = d0 + i;
+            if (d < D && abs(d - bestD) > 1) alt = imin(alt, my[i]);
+            if (d == bestD - 1) c0 = my[i];
+            if (d == bestD) c1 = my[i];
+            if (d == bestD + 1) c2 = my[i];
+        }
+        alt = gmin<Q>(alt);
+        if constexpr (Q > 1) {
+            c0 = gmin<Q>(c0);
+            c1 = gmin<Q>(c1);
+            c2 = gmin<Q>(c2);
+        }
     }
     if (sub != 0 || !ok) return;
 

@@ -394,6 +394,8 @@ def main() -> int:
                     help="skip the uneven-batch and serving legs")
     ap.add_argument("--serve-requests", type=int, default=400,
                     help="requests per arrival rate in the serving leg")
+    ap.add_argument("--only-batch", action="store_true",
+                    help="run only the 32-pair batched leg and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.derive_traffic:
         print(json.dumps(derive_traffic(*args.derive_traffic, args.calib_mib << 20), indent=1))
@@ -413,6 +415,10 @@ def main() -> int:
     sys.path.insert(0, here)
     import pyrsgm_amd as P
     from pyrsgm_amd import synth
+
+    if args.only_batch:
+        print(json.dumps(bench_batched(P, torch, log)), flush=True)
+        return 0
 
     W, H, D = 1920, 1080, 256
     pair = synth.config_pair("1920x1080_d256_census_l2")

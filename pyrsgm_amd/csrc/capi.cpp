@@ -9,6 +9,7 @@
 //   kPost      LR check + median + confidence clear + output conversions
 #include <math.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -187,6 +188,8 @@ hipError_t launchLevel(const LevelCall& c, Workspace& ws, hipStream_t s)
     a.numPaths = c.numPaths;
     a.exact = exactPathsNeeded(c);
     a.narrow = c.narrow;
+    static const int remap = getenv("PYRSGM_NO_XCD_REMAP") ? 0 : 1;
+    a.xcdRemap = remap;
     a.disp = ws.disp;
     a.conf = ws.conf;
     a.keys = ws.keys;

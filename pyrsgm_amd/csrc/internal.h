@@ -116,6 +116,7 @@ struct PathArgs {
     int numPaths;
     int exact;                 // pyramid: Cmax + 2*max(P2') may reach SHRT_MAX
     int narrow;                // byte volumes (see narrowVolumes)
+    int xcdRemap;              // XCD-contiguous line groups (on unless PYRSGM_NO_XCD_REMAP is set)
     int nLRBlocks;             // combined launch: blocks [0, nLR) run L->R,
     int nRLBlocks;             // [nLR, nLR+nRL) R->L, the rest T->B
     int16_t* disp;

@@ -815,6 +815,19 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
     }
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin by linear index, and
+// each XCD has its own L2.  T->B block b of n runs column group
+// xcdRemap(b, n): the blocks of one XCD (equal b % 8) get a contiguous range
+// of groups, so that neighbouring column groups, which read parts of the same
+// 128-byte lines at every step, share one L2 instead of each XCD fetching the
+// line.  A bijection of [0, n).  (Row paths read whole lines per chunk and keep
+// the dispatch order, which spreads their long lines evenly over the XCDs.)
+__device__ __forceinline__ int xcdRemap(int b, int n)
+{
+    const int q = n >> 3, r = n & 7, x = b & 7, i = b >> 3;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+}
+
 template <int G, int K, int MODE, bool ADAPT, bool EXACT, bool NARROW>
 __global__ __launch_bounds__(128) void kPaths(PathArgs a)
 {
@@ -831,9 +844,11 @@ __global__ __launch_bounds__(128) void kPaths(PathArgs a)
         runLine<G, K, MODE, ADAPT, EXACT, NARROW, kLR>(a, pr, b, s_lut, s_xchg, s_lds);
     else if (b < a.nLRBlocks + a.nRLBlocks)
         runLine<G, K, MODE, ADAPT, EXACT, NARROW, kRL>(a, pr, b - a.nLRBlocks, s_lut, s_xchg, s_lds);
-    else
-        runLine<G, K, MODE, ADAPT, EXACT, NARROW, kTB>(a, pr, b - a.nLRBlocks - a.nRLBlocks, s_lut,
-                                                       s_xchg, s_lds);
+    else {
+        const int t = b - a.nLRBlocks - a.nRLBlocks, nTB = (int)gridDim.x - a.nLRBlocks - a.nRLBlocks;
+        runLine<G, K, MODE, ADAPT, EXACT, NARROW, kTB>(a, pr, a.xcdRemap ? xcdRemap(t, nTB) : t,
+                                                       s_lut, s_xchg, s_lds);
+    }
 }
 
 inline int linesPerWave(int Dp) { return Dp <= 16 ? 4 : (Dp <= 32 ? 2 : 1); }

@@ -161,6 +161,24 @@ def bench_batched(P, torch, log, n=32, W=960, H=540, D=128):
     d16.fill_(0)
     bm = P.BatchMatcher(W, H, n, params)
     t_eager = _event_ms(torch, lambda: bm.compute(L, R, d16, d32, cf), reps=10)
+    # Batched roofline of the dominant launch (paths at 960x540x16 for all pairs):
+    # HIP events around that launch in eager batches.
+    Lb = P.lib()
+    Lb.pyrsgm_prof_start.argtypes = [C.c_int]
+    Lb.pyrsgm_prof_filter.argtypes = [C.c_int, C.c_int, C.c_int]
+    Lb.pyrsgm_prof_get.argtypes = [C.c_int] + [C.POINTER(C.c_int)] * 4 + [C.POINTER(C.c_float)]
+    Lb.pyrsgm_prof_filter(2, W, H)  # stage 2 = paths
+    Lb.pyrsgm_prof_start(64)
+    for _ in range(10):
+        bm.compute(L, R, d16, d32, cf)
+    torch.cuda.synchronize()
+    Lb.pyrsgm_prof_stop()
+    Lb.pyrsgm_prof_filter(-1, -1, -1)
+    st, w_, h_, l_, ms = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_float()
+    durs = [ms.value for i in range(Lb.pyrsgm_prof_count())
+            if Lb.pyrsgm_prof_get(i, C.byref(st), C.byref(w_), C.byref(h_), C.byref(l_), C.byref(ms)) == 0]
+    paths_us = 1000.0 * sum(durs) / len(durs) if durs else None
+    alg = n * algorithmic_bytes("paths", W, H, 16)
     bm.capture(L, R, d16, d32, cf)
     d16.fill_(0)
     t_graph = _event_ms(torch, bm.launch, reps=20)
@@ -174,7 +192,12 @@ def bench_batched(P, torch, log, n=32, W=960, H=540, D=128):
            "per_pair_ms": round(t_pp / n, 4),
            "speedup_vs_per_pair": round(t_pp / t_graph, 3),
            "bit_identical_to_per_pair": same,
-           "device_bytes": mem}
+           "device_bytes": mem,
+           "roofline_paths": None if not paths_us else {
+               "kernel": "paths", "level": f"{n}x{W}x{H}x16", "mean_us": round(paths_us, 2),
+               "alg_bytes_per_launch": alg, "achieved": round(alg / (paths_us * 1e-6) / 1e9, 1),
+               "peak": HBM_PEAK_GBS, "unit": "GB/s",
+               "frac": round(alg / (paths_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)}}
     log(f"[bench] batched: {out}")
     return out
 

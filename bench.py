@@ -576,7 +576,7 @@ def main() -> int:
             "disp32_mismatches": int((ref.disp32 != g32).sum()),
             "conf_mismatches": int((ref.conf != gc).sum()),
         }
-        log(f"[bench] cpu {c1 - c0:.2f} s/frame; parity {parity}")
+        log(f"[bench] cpu {(c1 - c0) / reps:.2f} s/frame ({reps} runs); parity {parity}")
 
     batched = None if args.no_batch else bench_batched(P, torch, log)
     uneven = serving = None

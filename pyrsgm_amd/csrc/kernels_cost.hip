@@ -56,17 +56,24 @@ __device__ __forceinline__ int btGrad(const uint8_t* img, int step, int W, int H
 
 // ---- descriptors + projection-key reset ----------------------------------
 // Census (sgm_kernels.cu:251-298): one 32-bit word per pixel and view, view 0
-// (left) then view 1 (right).  Block: 64 x 16 pixels, four rows per thread; the
-// 68 x 20 window they read (replicate-clamped) is loaded into registers with
-// every load in flight at once, then staged in LDS.
-constexpr int kCenRows = 16;
+// (left) then view 1 (right).  Block: 64 x 32 pixels, four horizontally
+// adjacent pixels in two rows per thread; the 68 x 36 window (replicate-clamped) is loaded
+// with every global load in flight at once and staged in LDS.  A thread reads
+// each window row of its four pixels as two dwords; each census bit is the
+// sign of (neighbour - centre), shifted in with one v_alignbit.
+constexpr int kCenRows = 32;
+
+__device__ __forceinline__ int byteOf(uint32_t lo, uint32_t hi, int b)
+{
+    return (int)(((b < 4 ? lo : hi) >> (8 * (b & 3))) & 0xFFu);
+}
 
 __global__ __launch_bounds__(256) void kPrepCensus(const uint8_t* left, int lstep,
                                                    const uint8_t* right, int rstep, uint32_t* cen,
                                                    uint32_t* keys, int W, int H, Batch b)
 {
-    constexpr int TW = 68, TH = kCenRows + 4, TILE = TW * TH, NL = (TILE + 255) / 256;
-    __shared__ uint8_t s_t[TH][TW];
+    constexpr int TW = 68, TS = 72, TH = kCenRows + 4, TILE = TW * TH, NL = (TILE + 255) / 256;
+    __shared__ __attribute__((aligned(16))) uint8_t s_t[TH][TS];
     const int z = blockIdx.z & 1;          // view
     const size_t pr = blockIdx.z >> 1;     // pair
     const uint8_t* img = z ? pairAt(right, pr, b.right) : pairAt(left, pr, b.left);
@@ -89,24 +96,48 @@ __global__ __launch_bounds__(256) void kPrepCensus(const uint8_t* left, int lste
         if (i < TILE) s_t[ty][tx] = (uint8_t)v[k];
     }
     __syncthreads();
-    const int lx = threadIdx.x & 63;
-    const int x = x0 + lx;
-    if (x >= W) return;
+    const int q = threadIdx.x & 15;
+    const int xg = x0 + 4 * q;
+    if (xg >= W) return;
 #pragma unroll
-    for (int r = 0; r < kCenRows / 4; ++r) {
-        const int ly = (threadIdx.x >> 6) + 4 * r;
+    for (int rg = 0; rg < kCenRows / 16; ++rg) {
+        const int ly = (threadIdx.x >> 4) + 16 * rg;
         const int y = y0 + ly;
         if (y >= H) break;
-        const int c = s_t[ly + 2][lx + 2];
-        uint32_t cv = 0;
+        uint32_t lo[5], hi[5];  // window columns 4q .. 4q+7 of rows ly .. ly+4
 #pragma unroll
         for (int dy = 0; dy < 5; ++dy) {
-#pragma unroll
-            for (int dx = 0; dx < 5; ++dx) cv = (cv << 1) | (uint32_t)(s_t[ly + dy][lx + dx] < c);
+            const uint32_t* r = reinterpret_cast<const uint32_t*>(&s_t[ly + dy][4 * q]);
+            lo[dy] = r[0];
+            hi[dy] = r[1];
         }
-        const size_t p = (size_t)y * W + x;
-        cen[(size_t)z * W * H + p] = cv;
-        if (z == 0) keys[p] = 0xFFFFFFFFu;
+        uint32_t cv[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int c = byteOf(lo[2], hi[2], j + 2);
+            uint32_t w = 0;
+#pragma unroll
+            for (int dy = 0; dy < 5; ++dy) {
+#pragma unroll
+                for (int dx = 0; dx < 5; ++dx)
+                    w = __builtin_amdgcn_alignbit(w, (uint32_t)(byteOf(lo[dy], hi[dy], j + dx) - c), 31);
+            }
+            cv[j] = w;
+        }
+        const size_t p = (size_t)y * W + xg;
+        uint32_t* const out = cen + (size_t)z * W * H + p;
+        if (xg + 3 < W && (((size_t)z * W * H + p) & 3) == 0) {  // whole, 16-byte aligned group
+            *reinterpret_cast<uint4*>(out) = make_uint4(cv[0], cv[1], cv[2], cv[3]);
+            if (z == 0) *reinterpret_cast<uint4*>(keys + p) = make_uint4(~0u, ~0u, ~0u, ~0u);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (xg + j < W) {
+                    out[j] = cv[j];
+                    if (z == 0) keys[p + j] = 0xFFFFFFFFu;
+                }
+            }
+        }
     }
 }
 

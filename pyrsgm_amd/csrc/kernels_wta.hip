@@ -83,34 +83,31 @@ __global__ __launch_bounds__(256) void kWTA(PathArgs a)
         const uint32_t wt[4] = {vt.x, vt.y, vt.z, vt.w};
         // Unused paths read 0xFF bytes: they pass the validity test, and their
         // 255 per label is taken off again by the 255 * 3 below.
-        uint32_t Sv[8];
-        const bool pad = d0 >= D;  // D is a multiple of 16: a lane is all labels or all padding
+        typedef uint16_t h2 __attribute__((ext_vector_type(2)));
+        auto H2 = [](uint32_t x) { return __builtin_bit_cast(h2, x); };
+        auto U32 = [](h2 x) { return __builtin_bit_cast(uint32_t, x); };
+        uint32_t Sv[8], Km[8];  // masked pair sums; 0xFFFF per field where all used paths are valid
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const uint32_t sel = (k & 1) ? 0x0C030C02u : 0x0C010C00u;  // bytes 2k, 2k+1 -> 16-bit fields
             const uint32_t a = __builtin_amdgcn_perm(0u, wr[k >> 1], sel);
             const uint32_t b = __builtin_amdgcn_perm(0u, wl[k >> 1], sel);
             const uint32_t c = __builtin_amdgcn_perm(0u, wt[k >> 1], sel);
-            const uint32_t sum = a + b + c;
-            typedef uint16_t h2 __attribute__((ext_vector_type(2)));
-            const h2 mn = __builtin_elementwise_min(__builtin_elementwise_min(
-                                                        __builtin_bit_cast(h2, a), __builtin_bit_cast(h2, b)),
-                                                    __builtin_bit_cast(h2, c));
-            const h2 one = __builtin_elementwise_min(mn, (h2){1, 1});
-            Sv[k] = pad ? 0u : __builtin_bit_cast(uint32_t, (h2)(__builtin_bit_cast(h2, sum) * one));
+            const h2 mn = __builtin_elementwise_min(__builtin_elementwise_min(H2(a), H2(b)), H2(c));
+            Km[k] = U32((h2){0, 0} - __builtin_elementwise_min(mn, (h2){1, 1}));
+            Sv[k] = (a + b + c) & Km[k];
         }
-        const int tot = 255 * 3;  // S counts 255 for every unused path
+        const bool pad = d0 >= D;  // D is a multiple of 16: a lane is all labels or all padding
+        const int tot = 255 * 3;   // S counts 255 for every unused path
         auto costOf = [&](uint32_t sf) { return sf ? tot - (int)sf : kMax; };
-        uint32_t best = 0;
+        // Packed keys S * 16 + (15 - label), 0 where invalid; their maximum.
+        h2 best2 = (h2){0, 0};
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            typedef uint16_t h2 __attribute__((ext_vector_type(2)));
-            const h2 sv = __builtin_bit_cast(h2, Sv[k]);
-            const h2 on = __builtin_elementwise_min(sv, (h2){1, 1});
-            const h2 key = (h2)(sv * (h2){16, 16}) + (h2)(on * (h2){(uint16_t)(15 - 2 * k), (uint16_t)(14 - 2 * k)});
-            const uint32_t kk = __builtin_bit_cast(uint32_t, key);
-            best = umax(best, umax(kk & 0xFFFFu, kk >> 16));
+            const uint32_t lab = (uint32_t)(15 - 2 * k) | ((uint32_t)(14 - 2 * k) << 16);
+            best2 = __builtin_elementwise_max(best2, H2((Sv[k] << 4) | (lab & Km[k])));
         }
+        const uint32_t best = umax(best2[0], best2[1]);
         if (pad)
             kmin = 0xFFFFFFFFu;
         else if (best == 0)
@@ -121,12 +118,15 @@ __global__ __launch_bounds__(256) void kWTA(PathArgs a)
         minCost = (int)(int16_t)(kmin >> 16);
         bestD = (int)(kmin & 0xFFFFu);
         const int bl = bestD - d0;  // best label's index in this lane (may be outside [0, 16))
-        // Field f of the packed sums (kMax outside this lane).
+        // Field f of the packed sums (kMax outside this lane), by a 3-level select
+        // tree on the pair index (a run-time index into Sv would put Sv in memory).
         auto field = [&](int f) {
-            if (f < 0 || f >= 16) return kMax;
-            uint32_t x = Sv[0];
-#pragma unroll
-            for (int k = 1; k < 8; ++k) x = (f >> 1) == k ? Sv[k] : x;
+            if (pad || f < 0 || f >= 16) return kMax;
+            const int k = f >> 1;
+            const uint32_t a01 = (k & 1) ? Sv[1] : Sv[0], a23 = (k & 1) ? Sv[3] : Sv[2];
+            const uint32_t a45 = (k & 1) ? Sv[5] : Sv[4], a67 = (k & 1) ? Sv[7] : Sv[6];
+            const uint32_t a03 = (k & 2) ? a23 : a01, a47 = (k & 2) ? a67 : a45;
+            const uint32_t x = (k & 4) ? a47 : a03;
             return costOf((f & 1) ? (x >> 16) : (x & 0xFFFFu));
         };
         c0 = field(bl - 1);
@@ -134,14 +134,15 @@ __global__ __launch_bounds__(256) void kWTA(PathArgs a)
         // alt: best sum outside labels bl-1 .. bl+1.
         const int sh = imin(imax(bl - 1, -3), 17);
         const uint32_t X = (sh >= 0 ? (7u << sh) : (7u >> (-sh))) & 0xFFFFu;
-        uint32_t bestAlt = 0;
+        h2 alt2 = (h2){0, 0};
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            const uint32_t m2 = (X >> (2 * k)) & 3u;
-            const uint32_t keep = ((m2 & 1u) ? 0u : 0xFFFFu) | ((m2 & 2u) ? 0u : 0xFFFF0000u);
-            const uint32_t v = Sv[k] & keep;
-            bestAlt = umax(bestAlt, umax(v & 0xFFFFu, v >> 16));
+            // 0xFFFF in the fields of excluded labels 2k, 2k+1.
+            const uint32_t ex = (__builtin_amdgcn_ubfe(X, 2 * k, 1) |
+                                 (__builtin_amdgcn_ubfe(X, 2 * k + 1, 1) << 16)) * 0xFFFFu;
+            alt2 = __builtin_elementwise_max(alt2, H2(Sv[k] & ~ex));
         }
+        const uint32_t bestAlt = pad ? 0u : umax(alt2[0], alt2[1]);
         alt = gmin<Q>(costOf(bestAlt));
         if constexpr (Q > 1) {
             c0 = gmin<Q>(c0);

@@ -157,10 +157,14 @@ hipError_t launchPaths(const PathArgs& a, int mode, int adaptive, hipStream_t s)
 
 hipError_t launchLevel(const LevelCall& c, Workspace& ws, hipStream_t s)
 {
-    profBegin(PYRSGM_STAGE_PREP, c.W, c.H, c.D, s);
-    hipError_t e = launchPrep(c, ws, s);
-    profEnd(s);
-    if (e != hipSuccess) return e;
+    hipError_t e = hipSuccess;
+    if (c.stages & kStagePrep) {
+        profBegin(PYRSGM_STAGE_PREP, c.W, c.H, c.D, s);
+        e = launchPrep(c, ws, s);
+        profEnd(s);
+        if (e != hipSuccess) return e;
+    }
+    if (!(c.stages & kStageRest)) return hipSuccess;
     profBegin(PYRSGM_STAGE_COST, c.W, c.H, c.D, s);
     e = launchCost(c, ws, s);
     profEnd(s);
@@ -288,6 +292,7 @@ int computeSingleScale(
     c.mode = kSingle;
     c.adaptive = buildP2Lut(P2, adaptiveP2, p2Alpha, p2Beta, p2LowerMult, p2UpperMult, c.lut);
     c.narrow = narrowVolumes(c);
+    c.stages = kStagesAll;
     c.batch = b;
     c.batch.ws = ws->slice;
     const hipError_t e = launchLevel(c, *ws, stream);
@@ -305,7 +310,7 @@ int computeLevel(
     int maxDisparity, int blockSize, int P1, int P2, int uniquenessRatio, int disp12MaxDiff,
     int preFilterCap, int costMode, int postFlags, int numPaths, int refineLevel,
     float minConfidence, int adaptiveP2, float p2Alpha, float p2Beta, float p2LowerMult,
-    float p2UpperMult, const Batch& b, hipStream_t stream)
+    float p2UpperMult, const Batch& b, hipStream_t stream, int stages)
 {
     if (!workspace || !d_left_gray || !d_right_gray || !geometryOk(width, height, numDisparities) ||
         (refineLevel && numDisparities > 256) || !paramsOk(blockSize, P1, P2))
@@ -358,6 +363,7 @@ int computeLevel(
     c.minConf = fminf(fmaxf(minConfidence, 0.0f), 1.0f);
     c.adaptive = buildP2Lut(P2, adaptiveP2, p2Alpha, p2Beta, p2LowerMult, p2UpperMult, c.lut);
     c.narrow = narrowVolumes(c);
+    c.stages = stages;
     c.batch = b;
     c.batch.ws = ws->slice;
     const hipError_t e = launchLevel(c, *ws, stream);

@@ -75,6 +75,7 @@ struct LevelCall {
     float minConf;
     int adaptive;
     int narrow;      // byte volumes (narrowVolumes)
+    int stages;      // kStagePrep | kStageRest
     Batch batch;
     int32_t lut[256];
 };
@@ -134,6 +135,11 @@ hipError_t launchPathsCoarse(const PathArgs& a, int adaptive, hipStream_t s);
 hipError_t launchPathsRefine(const PathArgs& a, int adaptive, hipStream_t s);
 hipError_t launchWTA(const PathArgs& a, int mode, hipStream_t s);
 
+// Parts of a level a call enqueues: the descriptor pass (census words / BT
+// intervals and the key reset, which read only the level's images) and the
+// rest (costs, paths, winner-take-all, post-processing).
+constexpr int kStagePrep = 1, kStageRest = 2, kStagesAll = 3;
+
 // The C-ABI level / single-scale computations for b.n pairs at once (the
 // extern "C" entry points call these with one pair).
 int computeSingleScale(void* workspace, const unsigned char* d_left_gray,
@@ -156,7 +162,8 @@ int computeLevel(void* workspace, const unsigned char* d_left_gray,
                  int blockSize, int P1, int P2, int uniquenessRatio, int disp12MaxDiff,
                  int preFilterCap, int costMode, int postFlags, int numPaths, int refineLevel,
                  float minConfidence, int adaptiveP2, float p2Alpha, float p2Beta,
-                 float p2LowerMult, float p2UpperMult, const Batch& b, hipStream_t stream);
+                 float p2LowerMult, float p2UpperMult, const Batch& b, hipStream_t stream,
+                 int stages = kStagesAll);
 // pyrDown of n pairs of (one or two) images; src / dst pair strides in bytes.
 hipError_t launchPyrDownBatch(const uint8_t* srcA, int sstepA, const uint8_t* srcB, int sstepB,
                               size_t sStride, int sw, int sh, uint8_t* dstA, int dstepA,

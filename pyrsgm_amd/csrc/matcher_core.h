@@ -92,6 +92,13 @@ struct MatcherCore {
     std::array<DeviceImage<float>, kMaxPyramidLevels + 1> conf;                // [0] scratch
     std::array<LevelInfo, kMaxPyramidLevels + 1> info{};
     size_t workspaceBytes = 0;
+    // The refinement levels' descriptor passes depend only on the pyramid images,
+    // so for several pairs compute() runs them on `side` while the coarse level
+    // runs on the caller's stream (fork / join through two events; graph-capture
+    // safe).  For one pair the cross-stream synchronisation costs more than the
+    // overlap saves (measured), and everything stays on the caller's stream.
+    hipStream_t side = nullptr;
+    hipEvent_t evFork = nullptr, evJoin = nullptr;
 
     // Reference: stereo_matcher.cpp:169-193.  Fills info[0..levels].
     static void geometry(int w, int h, const Params& params,
@@ -174,6 +181,11 @@ struct MatcherCore {
                 }
             }
             conf[0].create(w, h * n);
+            if (levels > 0 && n > 1) {
+                check(hipStreamCreateWithFlags(&side, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+                check(hipEventCreateWithFlags(&evFork, hipEventDisableTiming), "hipEventCreateWithFlags");
+                check(hipEventCreateWithFlags(&evJoin, hipEventDisableTiming), "hipEventCreateWithFlags");
+            }
         } catch (...) {
             destroy();
             throw;
@@ -184,6 +196,12 @@ struct MatcherCore {
 
     void destroy()
     {
+        if (side) (void)hipStreamSynchronize(side);
+        if (evFork) (void)hipEventDestroy(evFork);
+        if (evJoin) (void)hipEventDestroy(evJoin);
+        if (side) (void)hipStreamDestroy(side);
+        side = nullptr;
+        evFork = evJoin = nullptr;
         for (auto& ws : workspaces) {
             if (ws) pyrsgm_destroy_workspace(ws);
             ws = nullptr;
@@ -269,7 +287,27 @@ struct MatcherCore {
             pyr::profEnd(stream);
             if (e != hipSuccess) throwHip("pyrDown", e);
         }
+        // Fork: the refinement levels' descriptor passes on the side stream.
+        if (side) {
+            check(hipEventRecord(evFork, stream), "hipEventRecord");
+            check(hipStreamWaitEvent(side, evFork, 0), "hipStreamWaitEvent");
+            for (int level = levels - 1; level >= 0; --level) levelCall(level, L, R, d16, d32, oc, side, pyr::kStagePrep, count);
+            check(hipEventRecord(evJoin, side), "hipEventRecord");
+        }
         for (int level = levels; level >= 0; --level) {
+            if (side && level == levels - 1) check(hipStreamWaitEvent(stream, evJoin, 0), "hipStreamWaitEvent");
+            levelCall(level, L, R, d16, d32, oc, stream,
+                      (side && level < levels) ? pyr::kStageRest : pyr::kStagesAll, count);
+        }
+    }
+
+    // One pyramid level (or the given stages of it) for the first `count` pairs.
+    void levelCall(int level, BatchBuffer<const uint8_t> L, BatchBuffer<const uint8_t> R,
+                   BatchBuffer<int16_t> d16, BatchBuffer<float> d32, BatchBuffer<float> oc,
+                   hipStream_t stream, int stages, int count)
+    {
+        const int costMode = params.cost_mode == CostMode::Census ? PYRSGM_COST_CENSUS : PYRSGM_COST_BT;
+        {
             const LevelInfo& li = info[level];
             const bool refine = li.refine;
             const bool fine = level == 0;
@@ -303,7 +341,7 @@ struct MatcherCore {
                 params.disp12_max_diff, params.pre_filter_cap, costMode, postFlags(level),
                 params.num_paths, refine ? 1 : 0, params.pyramid_min_confidence,
                 params.adaptive_p2 ? 1 : 0, params.p2_alpha, params.p2_beta, params.p2_lower_mult,
-                params.p2_upper_mult, b, stream);
+                params.p2_upper_mult, b, stream, stages);
             if (ret != 0) {
                 std::ostringstream oss;
                 oss << "pyramid level " << level << " (" << li.width << "x" << li.height

@@ -219,6 +219,41 @@ __device__ __forceinline__ int gmax(int v)
     return v;
 }
 
+// gmax split before its last cross-row level: for 32- and 64-lane lines
+// gmaxPartial leaves, in every lane, the maximum over half of its line and
+// lastPair gives that value and the other half's (a permlane swap), so that
+// gmax(v) = max(a, b); the path step folds that level into its subtraction:
+// x - gmax(v) = min(x - a, x - b).  A 16-lane line keeps the whole reduction
+// in gmaxPartial (a = b): folding its last DPP level (row_mirror) into the
+// subtraction's v_subrev_u32_dpp gave wrong results on gfx950.
+template <int G>
+__device__ __forceinline__ int gmaxPartial(int v)
+{
+    v = imax(v, dmov<kDppQuadXor1>(v));
+    v = imax(v, dmov<kDppQuadXor2>(v));
+    v = imax(v, dmov<kDppRowHalfMirror>(v));
+    v = imax(v, dmov<kDppRowMirror>(v));
+    if constexpr (G >= 64) {
+        const auto s = __builtin_amdgcn_permlane16_swap((uint32_t)v, (uint32_t)v, false, false);
+        v = imax((int)s[0], (int)s[1]);
+    }
+    return v;
+}
+
+template <int G>
+__device__ __forceinline__ void lastPair(int p, int& a, int& b)
+{
+    if constexpr (G == 16) {
+        a = p;
+        b = p;
+    } else {
+        const auto s = G == 32 ? __builtin_amdgcn_permlane16_swap((uint32_t)p, (uint32_t)p, false, false)
+                               : __builtin_amdgcn_permlane32_swap((uint32_t)p, (uint32_t)p, false, false);
+        a = (int)s[0];
+        b = (int)s[1];
+    }
+}
+
 // Lane n of each 16-lane row, broadcast to the row (DPP row_newbcast).
 template <int N>
 __device__ __forceinline__ int rowBcast(int v)
@@ -586,7 +621,7 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
     int Hs[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) Hs[k] = 0;
-    int maxH = 0;                  // fast form: max over the line of Hs
+    int pmaxH = 0;                 // fast form: gmaxPartial of Hs (see lastPair)
     int minPrev = 0;               // EXACT
     bool active = false;           // EXACT
     int carryBase = a.minD;        // label-window base of the previous step (slot row)
@@ -672,16 +707,21 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
                     for (int k = 0; k < K; ++k) y[k] = imax(lm[k], lp[k]);
                 }
             }
+            // H'[d] + f and max(H'[d-1], H'[d+1]) + f - P1 share the term
+            // f = E - max_d H': Hc = max(max(h0, y - P1) + E - max H', E - P2'),
+            // and the last level of the line maximum folds into the subtraction.
+            int ma, mb;
+            lastPair<G>(pmaxH, ma, mb);
 #pragma unroll
             for (int k = 0; k < K; ++k) {
-                const int f = e[k] - maxH;
+                const int ze = imax(h0[k], y[k] - P1) + e[k];
                 const int t3 = (int)__builtin_elementwise_sub_sat((uint32_t)e[k], (uint32_t)p2);
-                Hc[k] = imax(imax(h0[k] + f, y[k] + f - P1), t3);
+                Hc[k] = imax(imin(ze - ma, ze - mb), t3);
             }
             int m = Hc[0];
 #pragma unroll
             for (int k = 1; k < K; ++k) m = imax(m, Hc[k]);
-            maxH = gmax<G>(m);
+            pmaxH = gmaxPartial<G>(m);
 #pragma unroll
             for (int k = 0; k < K; ++k) Hs[k] = Hc[k];
         } else {

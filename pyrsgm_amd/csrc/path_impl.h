@@ -347,11 +347,13 @@ struct ChunkAddr {
 //    step); vertical path [step][line][label].  A horizontal line's row and a
 //    vertical step's row are each one contiguous run of the path-cost volume,
 //    written with 16-byte stores.
-template <int G, int K, bool NARROW = false>
+template <int G, int K, bool NARROW = false, bool PK = false>
 struct Lds {
     static_assert(!NARROW || K == 1, "byte volumes have one label per lane");
+    static_assert(!PK || (NARROW && G == 16), "paired lines need byte volumes of 16 labels");
     static constexpr int U = prefetchSteps<K>();
     static constexpr int LPW = 64 / G;
+    static constexpr int LPWC = PK ? 2 * LPW : LPW;    // lines per wave (two per lane when paired)
     static constexpr int ESZ = NARROW ? 1 : 2;         // bytes per label in volumes and tiles
     static constexpr int VB = G * K * ESZ;             // bytes of one pixel's labels
     static constexpr int W32 = (K + 1) / 2;            // cost dwords per lane and step
@@ -359,22 +361,24 @@ struct Lds {
     // (K = 1, 32 bytes) or W32 dwords per step (64 bytes); read as CQ 16-byte
     // pieces.
     static constexpr bool PACK16 = K == 1 && !NARROW;
-    static constexpr int CQ = NARROW ? 1 : (PACK16 ? 2 : 4);
-    static_assert(U * (NARROW ? 1 : (PACK16 ? 2 : 4 * W32)) == CQ * 16,
+    static constexpr int CQ = NARROW ? (PK ? 2 : 1) : (PACK16 ? 2 : 4);
+    static_assert(U * (NARROW ? (PK ? 2 : 1) : (PACK16 ? 2 : 4 * W32)) == CQ * 16,
                   "a compute lane's chunk of costs is CQ pieces");
     // Bytes per compute lane in a cost slot: 16 bytes of padding keep the
     // lanes' 16-byte reads on distinct banks.  Byte slots are unpadded (16-byte
     // reads at a 16-byte stride are conflict-free) and each line's 16 lanes are
     // followed by LPAD bytes, which spreads the memory wave's byte writes of
     // different lines over different banks.
-    static constexpr int LANE = NARROW ? 16 : CQ * 16 + 16;
+    static constexpr int LANE = NARROW ? 16 * CQ : CQ * 16 + 16;
     static constexpr int LPAD = NARROW ? 16 : 0;
     static constexpr int CSLOT = 64 * LANE + LPW * LPAD;
     __device__ static constexpr int laneOff(int l) { return l * LANE + (l / G) * LPAD; }
-    static constexpr int WSLOT = 64 * 4;
+    // Per-step words, [line][step slot]: the memory wave writes 64 per lane set
+    // (two sets for paired lines).
+    static constexpr int WSLOT = 64 * 4 * (PK ? 2 : 1);
     static constexpr int ROWH = U * VB + VB;           // horizontal tile row, bytes
-    static constexpr int TSLOT = (LPW * ROWH > U * LPW * VB) ? LPW * ROWH : U * LPW * VB;
-    static constexpr int PIECES = LPW * U * VB / 16;   // 16-byte pieces per chunk
+    static constexpr int TSLOT = (LPWC * ROWH > U * LPWC * VB) ? LPWC * ROWH : U * LPWC * VB;
+    static constexpr int PIECES = LPWC * U * VB / 16;  // 16-byte pieces per chunk
     static_assert(PIECES % 64 == 0, "whole pieces per memory lane");
     static constexpr int NQ = PIECES / 64;
     static constexpr int OFF_W = 3 * CSLOT;
@@ -402,11 +406,12 @@ __device__ __forceinline__ void waveLdsFence()
 }
 
 // MODE: kSingle, kCoarse or kRefine.
-template <int G, int K, int MODE, bool ADAPT, bool EXACT, bool NARROW, int DIR>
+template <int G, int K, int MODE, bool ADAPT, bool EXACT, bool NARROW, bool PK, int DIR>
 __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int lineGroup,
                                         const int* s_lut, int* s_xchg, char* s_lds)
 {
     constexpr int LPW = 64 / G;
+    constexpr int LPWC = PK ? 2 * LPW : LPW;  // lines per wave
     constexpr int GK = G * K;
     constexpr int U = prefetchSteps<K>();
     static_assert(U <= 16, "per-step words use one 16-lane row slot per step");
@@ -415,7 +420,7 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
     constexpr bool PYR = MODE != kSingle;
     constexpr bool REFINE = MODE == kRefine;
     constexpr bool WORDS = REFINE || ADAPT;  // per-step words are loaded
-    using TL = Lds<G, K, NARROW>;
+    using TL = Lds<G, K, NARROW, PK>;
     constexpr int ESZ = TL::ESZ;
     constexpr int VB = GK * ESZ;  // bytes of one pixel's label vector
     constexpr int W32 = TL::W32;
@@ -427,8 +432,8 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
     const int r16 = lane & 15;
     const int W = a.W, H = a.H, D = a.D;
     const int nLines = VERT ? W : H;
-    const int line0 = lineGroup * LPW;
-    const int nValid = imin(LPW, nLines - line0);
+    const int line0 = lineGroup * LPWC;
+    const int nValid = imin(LPWC, nLines - line0);
     int s0 = 0, s1 = VERT ? H : W;
     if constexpr (!PYR && DIR == kLR) {
         s0 = imax(a.startX, 0);
@@ -481,10 +486,11 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
     // Piece pc (16 bytes) of a chunk: tile row pc / PPR (a horizontal line or a
     // vertical step), byte ob inside that row.  The same mapping addresses the
     // cost loads and the path-cost stores.
-    constexpr int PPR = VERT ? LPW * VB / 16 : U * VB / 16;
+    constexpr int PPR = VERT ? LPWC * VB / 16 : U * VB / 16;
+    constexpr int NW = PK ? 2 : 1;  // per-step word loads per memory lane
     struct MBuf {
         u4v q[TL::NQ];
-        int w;
+        int w[NW];
     };
     auto mload = [&](MBuf& m, int k) {
         const int sc = chunkStart(k);
@@ -507,14 +513,17 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
             // Lane (line jm, slot r) loads the word of step sc + r; slots past the
             // chunk or the line, and absent lines, read 0 (offset out of range).
             const ChunkAddr<U> cw = chunk(sc, wRow, wElem, wElem, full);
-            const int jm = lane >> 4, r = lane & 15;
-            const int rel = RL ? cw.sLast - sc - r : r;
-            const bool ok = jm < nValid && r < nSteps;
-            const uint32_t voff = ok ? (uint32_t)jm * lineStride(wRow, wElem) + (uint32_t)rel * wStep : kOffLimit;
             const char* base = REFINE ? reinterpret_cast<const char*>(labelsV) : reinterpret_cast<const char*>(imgV);
             const auto rw = rsrc(base + cw.origin, cw.nr);
-            m.w = REFINE ? (int)__builtin_amdgcn_raw_buffer_load_b32(rw, voff, 0, 0)
-                         : (int)__builtin_amdgcn_raw_buffer_load_b8(rw, voff, 0, 0);
+#pragma unroll
+            for (int i = 0; i < NW; ++i) {
+                const int jm = (lane >> 4) + 4 * i, r = lane & 15;
+                const int rel = RL ? cw.sLast - sc - r : r;
+                const bool ok = jm < nValid && r < nSteps;
+                const uint32_t voff = ok ? (uint32_t)jm * lineStride(wRow, wElem) + (uint32_t)rel * wStep : kOffLimit;
+                m.w[i] = REFINE ? (int)__builtin_amdgcn_raw_buffer_load_b32(rw, voff, 0, 0)
+                                : (int)__builtin_amdgcn_raw_buffer_load_b8(rw, voff, 0, 0);
+            }
         }
     };
     // Scatter a loaded chunk into cost slot k % 3: label d of step u goes to
@@ -535,7 +544,8 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
                 const int jj = VERT ? col : row;
                 const int u = VERT ? row : (RL ? nSteps - 1 - col : col);
                 const int d0 = (ob - col * VB) / ESZ;
-                char* const p = slot + TL::laneOff(jj * G + d0 / K);
+                // Paired lines: line jj goes to half jj / LPW of lane (jj % LPW, label).
+                char* const p = slot + TL::laneOff((jj % LPW) * G + d0 / K) + (jj / LPW) * 16;
                 const u4v q = m.q[i];
                 if constexpr (NARROW) {
 #pragma unroll
@@ -559,7 +569,11 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
                 }
             }
         }
-        if constexpr (WORDS) *reinterpret_cast<int*>(s_lds + TL::OFF_W + (k % 3) * TL::WSLOT + lane * 4) = m.w;
+        if constexpr (WORDS) {
+#pragma unroll
+            for (int i = 0; i < NW; ++i)
+                *reinterpret_cast<int*>(s_lds + TL::OFF_W + (k % 3) * TL::WSLOT + (64 * i + lane) * 4) = m.w[i];
+        }
     };
     // Write path-cost tile k % 2 to the volume; pieces of absent lines or of
     // steps past the line get an out-of-range offset and are dropped.
@@ -577,7 +591,7 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
             const int pc = lane + 64 * i;
             const int row = pc / PPR;
             const int ob = (pc - row * PPR) * 16;
-            const int tbyte = row * (VERT ? LPW * VB : TL::ROWH) + ob;
+            const int tbyte = row * (VERT ? LPWC * VB : TL::ROWH) + ob;
             const u4v v = *reinterpret_cast<const u4v*>(tile + tbyte);
             const bool ok = row < nRows && ob / VB < nCols;
             const uint32_t go = ok ? (uint32_t)row * volRow + (uint32_t)ob : kOffLimit;
@@ -767,15 +781,140 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
     struct CBuf {
         u4v q[TL::CQ];
         int w;
+        int w2;  // paired lines: the second line's word
     };
-    // This lane's costs of chunk k (and its row's per-step word) from LDS.
+    // This lane's costs of chunk k (and its row's per-step words) from LDS.
     auto cread = [&](CBuf& c, int k) {
         const char* const p = s_lds + (k % 3) * TL::CSLOT + TL::laneOff(lane);
 #pragma unroll
         for (int i = 0; i < TL::CQ; ++i) c.q[i] = *reinterpret_cast<const u4v*>(p + 16 * i);
-        if constexpr (WORDS)
-            c.w = *reinterpret_cast<const int*>(s_lds + TL::OFF_W + (k % 3) * TL::WSLOT +
-                                                (j * 16 + r16) * 4);
+        if constexpr (WORDS) {
+            const char* const wbase = s_lds + TL::OFF_W + (k % 3) * TL::WSLOT;
+            c.w = *reinterpret_cast<const int*>(wbase + (j * 16 + r16) * 4);
+            if constexpr (PK) c.w2 = *reinterpret_cast<const int*>(wbase + ((j + LPW) * 16 + r16) * 4);
+        }
+    };
+
+    // ---- paired lines (byte volumes, 16 labels): lane (j, g) holds label g of
+    // line j in bits 0-15 and of line j + LPW in bits 16-31 of every packed
+    // value; the step is the fast form above in 16-bit packed arithmetic.
+    typedef int16_t s2v __attribute__((ext_vector_type(2)));
+    typedef uint16_t u2v __attribute__((ext_vector_type(2)));
+    auto S2 = [](uint32_t x) { return __builtin_bit_cast(s2v, x); };
+    auto U2 = [](uint32_t x) { return __builtin_bit_cast(u2v, x); };
+    auto Wd = [](auto x) { return __builtin_bit_cast(uint32_t, x); };
+    auto pmax = [&](uint32_t x, uint32_t y) { return Wd(__builtin_elementwise_max(S2(x), S2(y))); };
+    auto psub = [&](uint32_t x, uint32_t y) { return Wd((s2v)(S2(x) - S2(y))); };
+    auto padd = [&](uint32_t x, uint32_t y) { return Wd((s2v)(S2(x) + S2(y))); };
+    auto bsel = [](uint32_t m, uint32_t x, uint32_t y) { return (x & m) | (y & ~m); };
+    uint32_t HsP = 0, maxHP = 0;
+    int carryBase2 = a.minD, carryIm42 = 0;
+    const uint32_t P1P = (uint32_t)(uint16_t)P1 | ((uint32_t)(uint16_t)P1 << 16);
+
+    auto stepPK = [&](auto uc, uint32_t e2, uint32_t p2, bool moved, uint32_t bigMask, int shvA,
+                      int shvB, char* tpA, char* tpB) {
+        constexpr int u = decltype(uc)::value;
+        const uint32_t lowN = (uint32_t)dmov<kDppRowShr1>((int)HsP);
+        const uint32_t upN = (uint32_t)dmov<kDppRowShl1>((int)HsP);
+        uint32_t h0 = HsP, y = pmax(lowN, upN);
+        if (REFINE && __builtin_expect(moved, 0)) {
+            const int shA = rowBcast<u>(shvA), shB = rowBcast<u>(shvB);
+            if (!((bigMask >> u) & 1u)) {
+                // Shifts of -1, 0 or +1: row shifts of the previous values, per half.
+                const uint32_t tm2 = (uint32_t)dmov<kDppRowShr2>((int)HsP);
+                const uint32_t tp2 = (uint32_t)dmov<kDppRowShl2>((int)HsP);
+                const uint32_t mN = (shA < 0 ? 0xFFFFu : 0u) | (shB < 0 ? 0xFFFF0000u : 0u);
+                const uint32_t mP = (shA > 0 ? 0xFFFFu : 0u) | (shB > 0 ? 0xFFFF0000u : 0u);
+                h0 = bsel(mN, lowN, bsel(mP, upN, HsP));
+                const uint32_t lm = bsel(mN, tm2, bsel(mP, HsP, lowN));
+                const uint32_t lp = bsel(mN, HsP, bsel(mP, tp2, upN));
+                y = pmax(lm, lp);
+            } else {
+                // Re-index each half through ds_bpermute (out-of-window labels read 0).
+                const int lane0 = j * G;
+                auto src = [&](int sm) { return (lane0 + imin(imax(sm, 0), D - 1)) << 2; };
+                auto pick = [](uint32_t x, bool vx, uint32_t z, bool vz) {
+                    return (vx ? (x & 0xFFFFu) : 0u) | (vz ? (z & 0xFFFF0000u) : 0u);
+                };
+                const int smA = g + shA, smB = g + shB;
+                const uint32_t a0 = (uint32_t)__builtin_amdgcn_ds_bpermute(src(smA), (int)HsP);
+                const uint32_t am = (uint32_t)__builtin_amdgcn_ds_bpermute(src(smA - 1), (int)HsP);
+                const uint32_t ap = (uint32_t)__builtin_amdgcn_ds_bpermute(src(smA + 1), (int)HsP);
+                const uint32_t b0 = (uint32_t)__builtin_amdgcn_ds_bpermute(src(smB), (int)HsP);
+                const uint32_t bm = (uint32_t)__builtin_amdgcn_ds_bpermute(src(smB - 1), (int)HsP);
+                const uint32_t bp = (uint32_t)__builtin_amdgcn_ds_bpermute(src(smB + 1), (int)HsP);
+                h0 = pick(a0, smA >= 0 && smA < D, b0, smB >= 0 && smB < D);
+                const uint32_t lm = pick(am, smA >= 1 && smA <= D, bm, smB >= 1 && smB <= D);
+                const uint32_t lp = pick(ap, smA >= -1 && smA < D - 1, bp, smB >= -1 && smB < D - 1);
+                y = pmax(lm, lp);
+            }
+        }
+        const uint32_t ze = padd(pmax(h0, psub(y, P1P)), e2);
+        const uint32_t t3 = Wd(__builtin_elementwise_sub_sat(U2(e2), U2(p2)));
+        const uint32_t Hc = pmax(psub(ze, maxHP), t3);
+        uint32_t m = Hc;
+        m = pmax(m, (uint32_t)dmov<kDppQuadXor1>((int)m));
+        m = pmax(m, (uint32_t)dmov<kDppQuadXor2>((int)m));
+        m = pmax(m, (uint32_t)dmov<kDppRowHalfMirror>((int)m));
+        m = pmax(m, (uint32_t)dmov<kDppRowMirror>((int)m));
+        maxHP = m;
+        HsP = Hc;
+        *reinterpret_cast<uint8_t*>(tpA) = (uint8_t)Hc;
+        *reinterpret_cast<uint8_t*>(tpB) = (uint8_t)(Hc >> 16);
+    };
+
+    auto processPK = [&](const CBuf& c, int k, bool full) {
+        const int sc = chunkStart(k);
+        int shvA = 0, shvB = 0;
+        uint32_t moveMask = 0, bigMask = 0, p2v = 0;
+        if constexpr (REFINE) {
+            const int baseA = (int)(int16_t)(c.w & 0xFFFF), baseB = (int)(int16_t)(c.w2 & 0xFFFF);
+            shvA = baseA - dpp<kDppRowShr1>(carryBase, baseA);
+            shvB = baseB - dpp<kDppRowShr1>(carryBase2, baseB);
+            auto fold = [](uint64_t b) {
+                const uint64_t f2 = b | (b >> 32);
+                return (uint32_t)(f2 | (f2 >> 16)) & 0xFFFFu;
+            };
+            moveMask = fold(__ballot(shvA != 0 || shvB != 0));
+            bigMask = fold(__ballot(shvA > 1 || shvA < -1 || shvB > 1 || shvB < -1));
+            carryBase = rowBcast<U - 1>(baseA);
+            carryBase2 = rowBcast<U - 1>(baseB);
+        }
+        if constexpr (ADAPT) {
+            auto lut = [&](int w, int& carry) {
+                const int im4 = REFINE ? (int)((uint32_t)w >> 22) : w << 2;
+                const int prevIm4 = dpp<kDppRowShr1>(carry, im4);
+                const uint32_t d4 = __builtin_amdgcn_sad_u16((uint32_t)im4, (uint32_t)prevIm4, 0u);
+                carry = rowBcast<U - 1>(im4);
+                return (uint32_t)*reinterpret_cast<const int*>(reinterpret_cast<const char*>(s_lut) + d4);
+            };
+            const uint32_t pa = lut(c.w, carryIm4), pb = lut(c.w2, carryIm42);
+            p2v = (pa & 0xFFFFu) | (pb << 16);
+        }
+        const uint32_t p2c = (uint32_t)(uint16_t)a.P2 | ((uint32_t)(uint16_t)a.P2 << 16);
+        const int nSteps = full ? U : imin(U, s1 - sc);
+        char* const tile = s_lds + TL::OFF_T + (k & 1) * TL::TSLOT;
+        auto run = [&](auto uc) {
+            constexpr int u = decltype(uc)::value;
+            if (full || u < nSteps) {
+                const uint32_t p2 = ADAPT ? (uint32_t)rowBcast<u>((int)p2v) : p2c;
+                const bool moved = REFINE && ((moveMask >> u) & 1u);
+                // Cost bytes of step u of both lines -> 16-bit fields.
+                constexpr uint32_t sel = 0x0C000C00u | ((4u + (u & 3)) << 16) | (u & 3);
+                const uint32_t e2 = __builtin_amdgcn_perm(c.q[1][u >> 2], c.q[0][u >> 2], sel);
+                int tbA, tbB;
+                if constexpr (VERT) {
+                    tbA = (u * LPWC + j) * VB + g;
+                    tbB = (u * LPWC + j + LPW) * VB + g;
+                } else {
+                    const int su = RL ? (full ? U - 1 - u : nSteps - 1 - u) : u;
+                    tbA = j * TL::ROWH + su * VB + g;
+                    tbB = (j + LPW) * TL::ROWH + su * VB + g;
+                }
+                stepPK(uc, e2, p2, moved, bigMask, shvA, shvB, tile + tbA, tile + tbB);
+            }
+        };
+        unrollSteps<U>(run);
     };
 
     // Process chunk k; full: all U steps exist.
@@ -833,6 +972,12 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
         unrollSteps<U>(run);
     };
 
+    auto proc = [&](const CBuf& c, int k, bool full) {
+        if constexpr (PK)
+            processPK(c, k, full);
+        else
+            process(c, k, full);
+    };
     // Chunk k+1 is read from LDS while chunk k is processed; barrier k+1 ends
     // chunk k (its tile complete, slot k free for the memory wave).
     wgBarrier();
@@ -841,15 +986,15 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
     int k = 0;
     for (; k + 2 <= nFull; k += 2) {
         cread(cB, k + 1);
-        process(cA, k, true);
+        proc(cA, k, true);
         wgBarrier();
         if (k + 2 < nChunks) cread(cA, k + 2);
-        process(cB, k + 1, true);
+        proc(cB, k + 1, true);
         wgBarrier();
     }
     for (; k < nChunks; ++k) {
         if (k + 1 < nChunks) cread(cB, k + 1);
-        process(cA, k, chunkFull(k));
+        proc(cA, k, chunkFull(k));
         wgBarrier();
         cA = cB;
     }
@@ -868,12 +1013,12 @@ __device__ __forceinline__ int xcdRemap(int b, int n)
     return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
 }
 
-template <int G, int K, int MODE, bool ADAPT, bool EXACT, bool NARROW>
+template <int G, int K, int MODE, bool ADAPT, bool EXACT, bool NARROW, bool PK = false>
 __global__ __launch_bounds__(128) void kPaths(PathArgs a)
 {
     __shared__ int s_lut[256];
     __shared__ int s_xchg[MODE == kRefine ? 64 * K : 1];
-    __shared__ __attribute__((aligned(16))) char s_lds[Lds<G, K, NARROW>::BYTES];
+    __shared__ __attribute__((aligned(16))) char s_lds[Lds<G, K, NARROW, PK>::BYTES];
     if constexpr (ADAPT) {
         for (int i = threadIdx.x; i < 256; i += 128) s_lut[i] = a.lut[i];
         __syncthreads();
@@ -881,17 +1026,21 @@ __global__ __launch_bounds__(128) void kPaths(PathArgs a)
     const int b = blockIdx.x;
     const size_t pr = blockIdx.y;  // pair
     if (b < a.nLRBlocks)
-        runLine<G, K, MODE, ADAPT, EXACT, NARROW, kLR>(a, pr, b, s_lut, s_xchg, s_lds);
+        runLine<G, K, MODE, ADAPT, EXACT, NARROW, PK, kLR>(a, pr, b, s_lut, s_xchg, s_lds);
     else if (b < a.nLRBlocks + a.nRLBlocks)
-        runLine<G, K, MODE, ADAPT, EXACT, NARROW, kRL>(a, pr, b - a.nLRBlocks, s_lut, s_xchg, s_lds);
+        runLine<G, K, MODE, ADAPT, EXACT, NARROW, PK, kRL>(a, pr, b - a.nLRBlocks, s_lut, s_xchg, s_lds);
     else {
         const int t = b - a.nLRBlocks - a.nRLBlocks, nTB = (int)gridDim.x - a.nLRBlocks - a.nRLBlocks;
-        runLine<G, K, MODE, ADAPT, EXACT, NARROW, kTB>(a, pr, a.xcdRemap ? xcdRemap(t, nTB) : t,
-                                                       s_lut, s_xchg, s_lds);
+        runLine<G, K, MODE, ADAPT, EXACT, NARROW, PK, kTB>(a, pr, a.xcdRemap ? xcdRemap(t, nTB) : t,
+                                                           s_lut, s_xchg, s_lds);
     }
 }
 
 inline int linesPerWave(int Dp) { return Dp <= 16 ? 4 : (Dp <= 32 ? 2 : 1); }
+
+// Launches of at least this many 4-line waves use paired lines (byte volumes,
+// 16 labels, pyramid levels): four per SIMD.
+constexpr size_t kPairedWaves = 4096;
 
 template <int MODE, bool ADAPT, bool EXACT, bool NARROW>
 hipError_t dispatchPaths(const PathArgs& a, hipStream_t s)
@@ -903,6 +1052,20 @@ hipError_t dispatchPaths(const PathArgs& a, hipStream_t s)
     const int nTB = a.numPaths >= 3 ? (a.W + lpw - 1) / lpw : 0;
     const int blocks = b.nLRBlocks + b.nRLBlocks + nTB;
     const dim3 grid(blocks, a.batch.n);
+    if constexpr (NARROW && MODE != kSingle) {
+        // Paired lines when the launch has at least four 4-line waves per SIMD
+        // of the 1024: the recurrence wave is then bound by VALU issue, and two
+        // lines per lane in 16-bit halves nearly halve its instructions per line.
+        if (a.Dp == 16 && (size_t)blocks * a.batch.n >= kPairedWaves) {
+            PathArgs c = a;
+            c.nLRBlocks = a.numPaths >= 2 ? (a.H + 7) / 8 : 0;
+            c.nRLBlocks = (a.H + 7) / 8;
+            const int nTB8 = a.numPaths >= 3 ? (a.W + 7) / 8 : 0;
+            const dim3 grid8(c.nLRBlocks + c.nRLBlocks + nTB8, a.batch.n);
+            kPaths<16, 1, MODE, ADAPT, EXACT, true, true><<<grid8, 128, 0, s>>>(c);
+            return hipGetLastError();
+        }
+    }
     if constexpr (NARROW) {
         switch (a.Dp) {
         case 16: kPaths<16, 1, MODE, ADAPT, EXACT, true><<<grid, 128, 0, s>>>(b); break;

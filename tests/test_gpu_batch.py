@@ -165,3 +165,27 @@ def test_batch_partial_counts():
             U.assert_same(f"graph rep {rep} pair {i}", g16[i].cpu().numpy(), want[i][0])
         assert bool((g16[3] == 1234).all())
     bm.close()
+
+
+@pytest.mark.parametrize("n,w,h,D,levels", [
+    (16, 480, 272, 64, 1),   # refinement level 480x272x16: 4,096 four-line waves -> paired lines
+    (32, 480, 272, 32, 1),   # coarse level 240x136x16 with 32 pairs -> paired lines
+])
+def test_batch_paired_lines_equal_single_pairs(n, w, h, D, levels):
+    """Launches with many lines run two lines per lane (16-bit halves); single
+    pairs never do.  Every pair must still equal its single-pair result."""
+    lefts, rights = pairs(n, w, h, D - 1, seed0=7000 + D)
+    params = P.validate_params(P.Params(num_disparities=D, pyramid_levels=levels,
+                                        cost_mode=P.CostMode.Census, p1=8, p2=32,
+                                        adaptive_p2=True))
+    L = batch_tensor(lefts, torch.uint8)
+    R = batch_tensor(rights, torch.uint8)
+    bm = P.BatchMatcher(w, h, n, params)
+    d16, d32, cf = run_batch(bm, L, R, n, h, w)
+    bm.close()
+    for i in (0, n // 2, n - 1):
+        want = U.matcher(lefts[i], rights[i], params)
+        U.assert_same(f"pair {i} disp16", d16[i].cpu().numpy(), want[0])
+        U.assert_same(f"pair {i} conf", cf[i].cpu().numpy(), want[2])
+    ref = O.matcher_compute(lefts[1], rights[1], params)
+    U.assert_same("pair 1 vs oracle", d16[1].cpu().numpy(), ref.disp16)

@@ -901,7 +901,10 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
                 const bool moved = REFINE && ((moveMask >> u) & 1u);
                 // Cost bytes of step u of both lines -> 16-bit fields.
                 constexpr uint32_t sel = 0x0C000C00u | ((4u + (u & 3)) << 16) | (u & 3);
-                const uint32_t e2 = __builtin_amdgcn_perm(c.q[1][u >> 2], c.q[0][u >> 2], sel);
+                // (CQ is 2 whenever this runs; the index only keeps the
+                // never-called non-paired instantiation in bounds.)
+                constexpr int q1 = TL::CQ > 1 ? 1 : 0;
+                const uint32_t e2 = __builtin_amdgcn_perm(c.q[q1][u >> 2], c.q[0][u >> 2], sel);
                 int tbA, tbB;
                 if constexpr (VERT) {
                     tbA = (u * LPWC + j) * VB + g;

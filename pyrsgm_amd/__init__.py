@@ -196,6 +196,21 @@ def lib() -> C.CDLL:
         L.pyrsgm_serve_reset_stats.argtypes = [vp]
         L.pyrsgm_serve_reset_stats.restype = None
         L.pyrsgm_serve_now_ns.restype = ll
+        L.pyrsgm_mixed_create.argtypes = [ci, C.POINTER(ci), C.POINTER(ci), C.POINTER(_ParamsC),
+                                          C.c_char_p, ci]
+        L.pyrsgm_mixed_create.restype = vp
+        L.pyrsgm_mixed_destroy.argtypes = [vp]
+        L.pyrsgm_mixed_destroy.restype = None
+        L.pyrsgm_mixed_compute.argtypes = [vp, C.POINTER(_PairIOC), vp, C.c_char_p, ci]
+        L.pyrsgm_mixed_capture.argtypes = [vp, C.POINTER(_PairIOC), C.c_char_p, ci]
+        L.pyrsgm_mixed_launch.argtypes = [vp, vp, C.c_char_p, ci]
+        L.pyrsgm_mixed_set_tuning.argtypes = [vp, C.POINTER(MixedTuning), C.c_char_p, ci]
+        L.pyrsgm_mixed_get_tuning.argtypes = [vp, C.POINTER(MixedTuning)]
+        L.pyrsgm_mixed_tune.argtypes = [vp, C.POINTER(_PairIOC), vp, ci, C.POINTER(MixedTuning),
+                                        C.POINTER(C.c_double), C.c_char_p, ci]
+        L.pyrsgm_mixed_plan.argtypes = [vp, C.POINTER(ll)]
+        L.pyrsgm_mixed_device_bytes.argtypes = [vp]
+        L.pyrsgm_mixed_device_bytes.restype = sz
         _lib = L
     return _lib
 
@@ -426,6 +441,123 @@ class BatchMatcher:
 
     def device_memory_bytes(self) -> int:
         return lib().pyrsgm_batch_device_bytes(self._h)
+
+
+class _PairIOC(C.Structure):
+    """pyrsgm_pair_io_c (include/pyrsgm/pyrsgm_mixed.h)."""
+    _fields_ = [("left", C.c_void_p), ("left_pitch", C.c_size_t),
+                ("right", C.c_void_p), ("right_pitch", C.c_size_t),
+                ("disp16", C.c_void_p), ("disp16_pitch", C.c_size_t),
+                ("disp32", C.c_void_p), ("disp32_pitch", C.c_size_t),
+                ("conf", C.c_void_p), ("conf_pitch", C.c_size_t)]
+
+
+class MixedTuning(C.Structure):
+    """pyrsgm_mixed_tuning_c: streams for the per-pair stages (1..4), resident
+    workgroups per CU of the queued path launches (0: occupancy), paired lines
+    (-1 by launch size, 0 off, 1 on)."""
+    _fields_ = [("branches", C.c_int), ("blocks_per_cu", C.c_int), ("paired", C.c_int)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class MixedBatch:
+    """pyrsgm::MixedBatch: pairs of different sizes and parameters in one fused
+    pass (include/pyrsgm/pyrsgm_mixed.h).
+
+    ``specs`` is a list of (width, height, Params).  Each pair's tensors are
+    2-D device tensors (uint8 in; int16 / float32 out)."""
+
+    def __init__(self, specs):
+        err = C.create_string_buffer(512)
+        n = len(specs)
+        ws = (C.c_int * n)(*[int(w) for w, _, _ in specs])
+        hs = (C.c_int * n)(*[int(h) for _, h, _ in specs])
+        ps = (_ParamsC * n)(*[_to_c(p) for _, _, p in specs])
+        h = lib().pyrsgm_mixed_create(n, ws, hs, ps, err, 512)
+        if not h:
+            msg = err.value.decode()
+            raise (ValueError if "pyrsgm:" in msg and "HIP" not in msg else RuntimeError)(msg)
+        self._h = h
+        self.specs = list(specs)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pyrsgm_mixed_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _io(self, pairs):
+        """pairs: one (left, right, disp16, disp32, conf) tuple of tensors / None per pair."""
+        if len(pairs) != len(self.specs):
+            raise ValueError(f"pyrsgm: expected {len(self.specs)} pairs, got {len(pairs)}")
+        arr = (_PairIOC * len(pairs))()
+        for i, ((w, h, _), tensors) in enumerate(zip(self.specs, pairs)):
+            vals = []
+            for t in tensors:
+                if t is not None and (t.dim() != 2 or t.shape[0] != h or t.shape[1] != w):
+                    raise ValueError(f"pyrsgm: pair {i}: expected [{h}, {w}] tensors, got {tuple(t.shape)}")
+                vals += list(_ptr(t))
+            for (name, _), v in zip(_PairIOC._fields_, vals):
+                setattr(arr[i], name, v)
+        return arr
+
+    def compute(self, pairs, stream=None):
+        import torch
+        s = stream if stream is not None else torch.cuda.current_stream()
+        err = C.create_string_buffer(512)
+        if lib().pyrsgm_mixed_compute(self._h, self._io(pairs), s.cuda_stream, err, 512) != 0:
+            raise RuntimeError(err.value.decode())
+
+    def capture(self, pairs):
+        """Record compute() on these tensors into a HIP graph; replay with launch()."""
+        err = C.create_string_buffer(512)
+        if lib().pyrsgm_mixed_capture(self._h, self._io(pairs), err, 512) != 0:
+            raise RuntimeError(err.value.decode())
+
+    def launch(self, stream=None):
+        import torch
+        s = stream if stream is not None else torch.cuda.current_stream()
+        err = C.create_string_buffer(512)
+        if lib().pyrsgm_mixed_launch(self._h, s.cuda_stream, err, 512) != 0:
+            raise RuntimeError(err.value.decode())
+
+    def set_tuning(self, branches=4, blocks_per_cu=0, paired=-1):
+        err = C.create_string_buffer(512)
+        t = MixedTuning(branches, blocks_per_cu, paired)
+        if lib().pyrsgm_mixed_set_tuning(self._h, C.byref(t), err, 512) != 0:
+            raise ValueError(err.value.decode())
+
+    def tuning(self) -> dict:
+        t = MixedTuning()
+        lib().pyrsgm_mixed_get_tuning(self._h, C.byref(t))
+        return t.as_dict()
+
+    def tune(self, pairs, reps=3, stream=None):
+        """Time the candidate launch shapes; keeps the fastest.  Returns (tuning, ms per pass)."""
+        import torch
+        s = stream if stream is not None else torch.cuda.current_stream()
+        err = C.create_string_buffer(512)
+        best, ms = MixedTuning(), C.c_double()
+        if lib().pyrsgm_mixed_tune(self._h, self._io(pairs), s.cuda_stream, reps, C.byref(best),
+                                   C.byref(ms), err, 512) != 0:
+            raise RuntimeError(err.value.decode())
+        return best.as_dict(), ms.value
+
+    def plan(self) -> dict:
+        out = (C.c_longlong * 4)()
+        lib().pyrsgm_mixed_plan(self._h, out)
+        return {"queued_path_launches": out[0], "path_tasks": out[1], "work_items": out[2],
+                "per_pair_stage_calls": out[3]}
+
+    def device_memory_bytes(self) -> int:
+        return lib().pyrsgm_mixed_device_bytes(self._h)
 
 
 class Server:

@@ -23,6 +23,9 @@ SOURCES = [
     "kernels_path_refine.hip",
     "kernels_path_coarse.hip",
     "kernels_path_single.hip",
+    "kernels_pathq_refine.hip",
+    "kernels_pathq_coarse.hip",
+    "kernels_pathq_single.hip",
     "kernels_wta.hip",
     "kernels_cost.hip",
     "kernels_post.hip",
@@ -31,6 +34,7 @@ SOURCES = [
     "stereo_matcher.cpp",
     "matcher_capi.cpp",
     "server.cpp",
+    "mixed.cpp",
     "prof.cpp",
 ]
 

@@ -155,6 +155,30 @@ hipError_t launchPaths(const PathArgs& a, int mode, int adaptive, hipStream_t s)
     return launchPathsSingle(a, adaptive, s);
 }
 
+int queuedLinesPerBlock(int Dp, int paired)
+{
+    if (paired && Dp == 16) return 8;
+    return Dp <= 16 ? 4 : (Dp <= 32 ? 2 : 1);
+}
+
+int pairedLinesAuto(size_t blocks4) { return blocks4 >= kPairedWaves ? 1 : 0; }
+
+int queuedBlocksPerCU(int mode, int Dp, int adaptive, int exact, int narrow, int paired)
+{
+    if (mode == kRefine) return queuedOccupancyRefine(Dp, adaptive, exact, narrow, paired);
+    if (mode == kCoarse) return queuedOccupancyCoarse(Dp, adaptive, exact, narrow, paired);
+    return queuedOccupancySingle(Dp, adaptive, exact, narrow, paired);
+}
+
+hipError_t launchPathsQueued(const QueueArgs& q, int mode, int Dp, int adaptive, int exact,
+                             int narrow, int paired, int grid, hipStream_t s)
+{
+    if (q.total <= 0) return hipSuccess;
+    if (mode == kRefine) return launchPathsQueuedRefine(q, Dp, adaptive, exact, narrow, paired, grid, s);
+    if (mode == kCoarse) return launchPathsQueuedCoarse(q, Dp, adaptive, exact, narrow, paired, grid, s);
+    return launchPathsQueuedSingle(q, Dp, adaptive, narrow, grid, s);
+}
+
 hipError_t launchLevel(const LevelCall& c, Workspace& ws, hipStream_t s)
 {
     hipError_t e = hipSuccess;
@@ -164,11 +188,13 @@ hipError_t launchLevel(const LevelCall& c, Workspace& ws, hipStream_t s)
         profEnd(s);
         if (e != hipSuccess) return e;
     }
-    if (!(c.stages & kStageRest)) return hipSuccess;
-    profBegin(PYRSGM_STAGE_COST, c.W, c.H, c.D, s);
-    e = launchCost(c, ws, s);
-    profEnd(s);
-    if (e != hipSuccess) return e;
+    if (c.stages & kStageCost) {
+        profBegin(PYRSGM_STAGE_COST, c.W, c.H, c.D, s);
+        e = launchCost(c, ws, s);
+        profEnd(s);
+        if (e != hipSuccess) return e;
+    }
+    if (!(c.stages & (kStagePaths | kStageFinish))) return hipSuccess;
 
     PathArgs a;
     memset(&a, 0, sizeof(a));
@@ -202,10 +228,17 @@ hipError_t launchLevel(const LevelCall& c, Workspace& ws, hipStream_t s)
     a.batch = c.batch;
     memcpy(a.lut, c.lut, sizeof(a.lut));
 
-    profBegin(PYRSGM_STAGE_PATHS, c.W, c.H, c.D, s);
-    e = launchPaths(a, c.mode, c.adaptive, s);
-    profEnd(s);
-    if (e != hipSuccess) return e;
+    if (c.stages & kStagePaths) {
+        if (c.sink) {
+            c.sink->tasks.push_back(PathTask{a, c.mode, c.adaptive});
+        } else {
+            profBegin(PYRSGM_STAGE_PATHS, c.W, c.H, c.D, s);
+            e = launchPaths(a, c.mode, c.adaptive, s);
+            profEnd(s);
+            if (e != hipSuccess) return e;
+        }
+    }
+    if (!(c.stages & kStageFinish)) return hipSuccess;
     profBegin(PYRSGM_STAGE_WTA, c.W, c.H, c.D, s);
     e = launchWTA(a, c.mode, s);
     profEnd(s);
@@ -252,7 +285,8 @@ int computeSingleScale(
     size_t conf_float_step_bytes, int width, int height, int numDisparities, int minDisparity,
     int blockSize, int P1, int P2, int uniquenessRatio, int disp12MaxDiff, int preFilterCap,
     int costMode, int postFlags, int numPaths, int adaptiveP2, float p2Alpha, float p2Beta,
-    float p2LowerMult, float p2UpperMult, const Batch& b, hipStream_t stream)
+    float p2LowerMult, float p2UpperMult, const Batch& b, hipStream_t stream, int stages,
+    PathSink* sink)
 {
     if (!workspace || !d_left_gray || !d_right_gray || !geometryOk(width, height, numDisparities) ||
         !paramsOk(blockSize, P1, P2))
@@ -292,7 +326,8 @@ int computeSingleScale(
     c.mode = kSingle;
     c.adaptive = buildP2Lut(P2, adaptiveP2, p2Alpha, p2Beta, p2LowerMult, p2UpperMult, c.lut);
     c.narrow = narrowVolumes(c);
-    c.stages = kStagesAll;
+    c.stages = stages;
+    c.sink = sink;
     c.batch = b;
     c.batch.ws = ws->slice;
     const hipError_t e = launchLevel(c, *ws, stream);
@@ -310,7 +345,7 @@ int computeLevel(
     int maxDisparity, int blockSize, int P1, int P2, int uniquenessRatio, int disp12MaxDiff,
     int preFilterCap, int costMode, int postFlags, int numPaths, int refineLevel,
     float minConfidence, int adaptiveP2, float p2Alpha, float p2Beta, float p2LowerMult,
-    float p2UpperMult, const Batch& b, hipStream_t stream, int stages)
+    float p2UpperMult, const Batch& b, hipStream_t stream, int stages, PathSink* sink)
 {
     if (!workspace || !d_left_gray || !d_right_gray || !geometryOk(width, height, numDisparities) ||
         (refineLevel && numDisparities > 256) || !paramsOk(blockSize, P1, P2))
@@ -364,6 +399,7 @@ int computeLevel(
     c.adaptive = buildP2Lut(P2, adaptiveP2, p2Alpha, p2Beta, p2LowerMult, p2UpperMult, c.lut);
     c.narrow = narrowVolumes(c);
     c.stages = stages;
+    c.sink = sink;
     c.batch = b;
     c.batch.ws = ws->slice;
     const hipError_t e = launchLevel(c, *ws, stream);

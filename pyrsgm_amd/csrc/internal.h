@@ -5,7 +5,11 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <vector>
+
 namespace pyr {
+
+struct PathSink;
 
 enum Mode : int { kSingle = 0, kCoarse = 1, kRefine = 2 };
 
@@ -75,7 +79,8 @@ struct LevelCall {
     float minConf;
     int adaptive;
     int narrow;      // byte volumes (narrowVolumes)
-    int stages;      // kStagePrep | kStageRest
+    int stages;      // kStage* bits
+    PathSink* sink;  // non-null: path launches are recorded here instead of issued
     Batch batch;
     int32_t lut[256];
 };
@@ -136,9 +141,69 @@ hipError_t launchPathsRefine(const PathArgs& a, int adaptive, hipStream_t s);
 hipError_t launchWTA(const PathArgs& a, int mode, hipStream_t s);
 
 // Parts of a level a call enqueues: the descriptor pass (census words / BT
-// intervals and the key reset, which read only the level's images) and the
-// rest (costs, paths, winner-take-all, post-processing).
-constexpr int kStagePrep = 1, kStageRest = 2, kStagesAll = 3;
+// intervals and the key reset, which read only the level's images), the cost
+// volume, the path recurrences, and the finish (winner-take-all and
+// post-processing).
+constexpr int kStagePrep = 1, kStageCost = 2, kStagePaths = 4, kStageFinish = 8;
+constexpr int kStageRest = kStageCost | kStagePaths | kStageFinish;
+constexpr int kStagesAll = kStagePrep | kStageRest;
+
+// A path launch recorded instead of issued (LevelCall::sink): its arguments
+// for one pair (pair strides unused) and its kernel specialisation.
+struct PathTask {
+    PathArgs a;
+    int mode, adaptive;
+};
+struct PathSink {
+    std::vector<PathTask> tasks;
+};
+
+// ---- queued path launches (many pairs' levels of one specialisation) -----
+// Work item = (task, block) where a task's blocks are numbered as in its own
+// launch (L->R rows, R->L rows, T->B column groups); items are numbered task by
+// task, `start[t]` being task t's first.  Workgroups take items from
+// counter[0] with an atomic add until it passes the total; the last workgroup
+// to finish puts both counters back to zero, so consecutive launches on one
+// stream reuse them (launches that share counters must not overlap).
+struct QueueArgs {
+    const PathArgs* tasks;  // device table [nTasks] (nLRBlocks / nRLBlocks set per task)
+    const int* start;       // device [nTasks + 1]
+    uint32_t* counter;      // device [2], zero before the launch
+    int nTasks, total;
+};
+// Lines per block of a queued launch of labels Dp (paired: 16-label byte
+// volumes with two lines per lane).
+int queuedLinesPerBlock(int Dp, int paired);
+// Whether a launch of `blocks4` four-line blocks of one 16-label byte-volume
+// pyramid specialisation pairs lines (the per-launch rule of the path kernels).
+int pairedLinesAuto(size_t blocks4);
+// Path launches of at least this many 4-line waves use paired lines (byte
+// volumes, 16 labels, pyramid levels): four per SIMD.
+constexpr size_t kPairedWaves = 4096;
+// Workgroups of one queued launch that are resident at once per CU (occupancy).
+int queuedBlocksPerCU(int mode, int Dp, int adaptive, int exact, int narrow, int paired);
+int queuedOccupancySingle(int Dp, int adaptive, int exact, int narrow, int paired);
+int queuedOccupancyCoarse(int Dp, int adaptive, int exact, int narrow, int paired);
+int queuedOccupancyRefine(int Dp, int adaptive, int exact, int narrow, int paired);
+hipError_t launchPathsQueued(const QueueArgs& q, int mode, int Dp, int adaptive, int exact,
+                             int narrow, int paired, int grid, hipStream_t s);
+hipError_t launchPathsQueuedSingle(const QueueArgs& q, int Dp, int adaptive, int narrow, int grid,
+                                   hipStream_t s);
+hipError_t launchPathsQueuedCoarse(const QueueArgs& q, int Dp, int adaptive, int exact,
+                                   int narrow, int paired, int grid, hipStream_t s);
+hipError_t launchPathsQueuedRefine(const QueueArgs& q, int Dp, int adaptive, int exact,
+                                   int narrow, int paired, int grid, hipStream_t s);
+// Sets tasks[idx[i]].img / .istep (the per-call left image of single-scale
+// tasks with adaptive P2) on the device.
+constexpr int kMaxImgPatches = 64;
+struct ImgPatch {
+    PathArgs* tasks;
+    int n;
+    int idx[kMaxImgPatches];
+    int istep[kMaxImgPatches];
+    const uint8_t* img[kMaxImgPatches];
+};
+hipError_t launchImgPatch(const ImgPatch& p, hipStream_t s);
 
 // The C-ABI level / single-scale computations for b.n pairs at once (the
 // extern "C" entry points call these with one pair).
@@ -150,7 +215,8 @@ int computeSingleScale(void* workspace, const unsigned char* d_left_gray,
                        int minDisparity, int blockSize, int P1, int P2, int uniquenessRatio,
                        int disp12MaxDiff, int preFilterCap, int costMode, int postFlags,
                        int numPaths, int adaptiveP2, float p2Alpha, float p2Beta,
-                       float p2LowerMult, float p2UpperMult, const Batch& b, hipStream_t stream);
+                       float p2LowerMult, float p2UpperMult, const Batch& b, hipStream_t stream,
+                       int stages = kStagesAll, PathSink* sink = nullptr);
 int computeLevel(void* workspace, const unsigned char* d_left_gray,
                  const unsigned char* d_right_gray, const short* d_parent_disp,
                  const float* d_parent_conf, short* d_disp, float* d_disp_float,
@@ -163,7 +229,7 @@ int computeLevel(void* workspace, const unsigned char* d_left_gray,
                  int preFilterCap, int costMode, int postFlags, int numPaths, int refineLevel,
                  float minConfidence, int adaptiveP2, float p2Alpha, float p2Beta,
                  float p2LowerMult, float p2UpperMult, const Batch& b, hipStream_t stream,
-                 int stages = kStagesAll);
+                 int stages = kStagesAll, PathSink* sink = nullptr);
 // pyrDown of n pairs of (one or two) images; src / dst pair strides in bytes.
 hipError_t launchPyrDownBatch(const uint8_t* srcA, int sstepA, const uint8_t* srcB, int sstepB,
                               size_t sStride, int sw, int sh, uint8_t* dstA, int dstepA,

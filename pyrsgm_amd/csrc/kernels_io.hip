@@ -1,5 +1,6 @@
 // kernels_io.hip -- batched 2-D copies between caller buffers and a serving
-// lane's stacked staging buffers (one launch for any number of copies).
+// lane's stacked staging buffers (one launch for any number of copies), and
+// the per-call image patch of a queued path launch's task table.
 #include "device_common.h"
 #include "internal.h"
 
@@ -27,7 +28,25 @@ __global__ __launch_bounds__(256) void kCopy2D(const CopyDesc* descs)
     }
 }
 
+// Thread i < p.n points task p.idx[i] at its call's left image.
+__global__ __launch_bounds__(64) void kImgPatch(ImgPatch p)
+{
+    const int i = threadIdx.x;
+    if (i >= p.n) return;
+    PathArgs* t = p.tasks + p.idx[i];
+    t->img = p.img[i];
+    t->istep = p.istep[i];
+}
+
 } // namespace
+
+hipError_t launchImgPatch(const ImgPatch& p, hipStream_t s)
+{
+    if (p.n <= 0) return hipSuccess;
+    if (p.n > kMaxImgPatches) return hipErrorInvalidValue;
+    kImgPatch<<<1, 64, 0, s>>>(p);
+    return hipGetLastError();
+}
 
 hipError_t launchCopy2D(const CopyDesc* descs, int count, int maxRowBytes, int maxRows,
                         hipStream_t stream)

@@ -1,5 +1,6 @@
-// matcher_capi.cpp -- plain-C handles on pyrsgm::StereoMatcher (pyrsgm_py.h)
-// and pyrsgm::BatchMatcher (pyrsgm_batch.h).
+// matcher_capi.cpp -- plain-C handles on pyrsgm::StereoMatcher (pyrsgm_py.h),
+// pyrsgm::BatchMatcher (pyrsgm_batch.h), pyrsgm::Server (pyrsgm_serve.h) and
+// pyrsgm::MixedBatch (pyrsgm_mixed.h).
 #include <stdio.h>
 #include <string.h>
 
@@ -8,6 +9,7 @@
 
 #include "pyrsgm/pyrsgm.h"
 #include "pyrsgm/pyrsgm_batch.h"
+#include "pyrsgm/pyrsgm_mixed.h"
 #include "pyrsgm/pyrsgm_py.h"
 #include "pyrsgm/pyrsgm_serve.h"
 
@@ -343,5 +345,126 @@ void pyrsgm_serve_reset_stats(void* server)
 }
 
 long long pyrsgm_serve_now_ns(void) { return (long long)pyrsgm::Server::nowNs(); }
+
+// ---- MixedBatch -------------------------------------------------------------
+
+void* pyrsgm_mixed_create(int n, const int* widths, const int* heights,
+                          const pyrsgm_params_c* params, char* err, int errlen)
+{
+    if (n < 1 || !widths || !heights || !params) {
+        copyErr("pyrsgm: MixedBatch needs n >= 1 sizes and parameter sets", err, errlen);
+        return nullptr;
+    }
+    try {
+        std::vector<pyrsgm::PairSpec> specs((size_t)n);
+        for (int i = 0; i < n; ++i) {
+            specs[(size_t)i].width = widths[i];
+            specs[(size_t)i].height = heights[i];
+            specs[(size_t)i].params = toParams(params[i]);
+        }
+        return new pyrsgm::MixedBatch(specs);
+    } catch (const std::exception& e) {
+        copyErr(e.what(), err, errlen);
+    } catch (...) {
+        copyErr("pyrsgm: unknown error", err, errlen);
+    }
+    return nullptr;
+}
+
+void pyrsgm_mixed_destroy(void* batch) { delete static_cast<pyrsgm::MixedBatch*>(batch); }
+
+int pyrsgm_mixed_compute(void* batch, const pyrsgm_pair_io_c* io, hipStream_t stream, char* err,
+                         int errlen)
+{
+    if (!batch) return -1;
+    try {
+        static_cast<pyrsgm::MixedBatch*>(batch)->compute(io, stream);
+        return 0;
+    } catch (const std::exception& e) {
+        copyErr(e.what(), err, errlen);
+    }
+    return -1;
+}
+
+int pyrsgm_mixed_capture(void* batch, const pyrsgm_pair_io_c* io, char* err, int errlen)
+{
+    if (!batch) return -1;
+    try {
+        static_cast<pyrsgm::MixedBatch*>(batch)->capture(io);
+        return 0;
+    } catch (const std::exception& e) {
+        copyErr(e.what(), err, errlen);
+    }
+    return -1;
+}
+
+int pyrsgm_mixed_launch(void* batch, hipStream_t stream, char* err, int errlen)
+{
+    if (!batch) return -1;
+    try {
+        static_cast<pyrsgm::MixedBatch*>(batch)->launch(stream);
+        return 0;
+    } catch (const std::exception& e) {
+        copyErr(e.what(), err, errlen);
+    }
+    return -1;
+}
+
+int pyrsgm_mixed_set_tuning(void* batch, const pyrsgm_mixed_tuning_c* t, char* err, int errlen)
+{
+    if (!batch || !t) return -1;
+    try {
+        pyrsgm::MixedTuning m;
+        m.branches = t->branches;
+        m.blocks_per_cu = t->blocks_per_cu;
+        m.paired = t->paired;
+        static_cast<pyrsgm::MixedBatch*>(batch)->setTuning(m);
+        return 0;
+    } catch (const std::exception& e) {
+        copyErr(e.what(), err, errlen);
+    }
+    return -1;
+}
+
+int pyrsgm_mixed_get_tuning(void* batch, pyrsgm_mixed_tuning_c* out)
+{
+    if (!batch || !out) return -1;
+    const pyrsgm::MixedTuning m = static_cast<pyrsgm::MixedBatch*>(batch)->tuning();
+    out->branches = m.branches;
+    out->blocks_per_cu = m.blocks_per_cu;
+    out->paired = m.paired;
+    return 0;
+}
+
+int pyrsgm_mixed_tune(void* batch, const pyrsgm_pair_io_c* io, hipStream_t stream, int reps,
+                      pyrsgm_mixed_tuning_c* best, double* best_ms, char* err, int errlen)
+{
+    if (!batch) return -1;
+    try {
+        const pyrsgm::MixedTuning m = static_cast<pyrsgm::MixedBatch*>(batch)->tune(io, stream, reps, best_ms);
+        if (best) {
+            best->branches = m.branches;
+            best->blocks_per_cu = m.blocks_per_cu;
+            best->paired = m.paired;
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        copyErr(e.what(), err, errlen);
+    }
+    return -1;
+}
+
+int pyrsgm_mixed_plan(void* batch, long long out[4])
+{
+    if (!batch || !out) return -1;
+    const std::vector<long long> p = static_cast<pyrsgm::MixedBatch*>(batch)->plan();
+    for (int i = 0; i < 4; ++i) out[i] = p[(size_t)i];
+    return 0;
+}
+
+size_t pyrsgm_mixed_device_bytes(void* batch)
+{
+    return batch ? static_cast<pyrsgm::MixedBatch*>(batch)->deviceMemoryBytes() : 0;
+}
 
 } // extern "C"

@@ -232,28 +232,63 @@ struct MatcherCore {
         if (count < 1 || count > n) throw std::invalid_argument("pyrsgm: pair count outside [1, batch]");
         if (!L.data || !R.data) throw std::invalid_argument("pyrsgm: null input image");
         if (!d16.data && !d32.data && !cf.data) return;
-        BatchBuffer<float> oc = cf;
-        if (!oc.data && d32.data) oc = {conf[0].data, conf[0].pitch, stride(conf[0], 0)};
-        const int costMode = params.cost_mode == CostMode::Census ? PYRSGM_COST_CENSUS : PYRSGM_COST_BT;
+        const BatchBuffer<float> oc = confRoute(d32, cf);
         if (levels == 0) {
-            pyr::Batch b;
-            b.n = count;
-            b.left = L.pair_stride;
-            b.right = R.pair_stride;
-            b.out16 = d16.pair_stride;
-            b.out32 = d32.pair_stride;
-            b.outc = oc.pair_stride;
-            const int ret = pyr::computeSingleScale(
-                workspaces[0], L.data, R.data, d16.data, d32.data, oc.data, L.pitch, R.pitch,
-                d16.pitch, d32.pitch, oc.pitch, width, height, params.num_disparities,
-                params.min_disparity, params.block_size, params.p1, params.p2,
-                params.uniqueness_ratio, params.disp12_max_diff, params.pre_filter_cap, costMode,
-                postFlags(0), params.num_paths, params.adaptive_p2 ? 1 : 0, params.p2_alpha,
-                params.p2_beta, params.p2_lower_mult, params.p2_upper_mult, b, stream);
-            if (ret != 0) throwHip("single-scale SGM", hipGetLastError());
+            singleCall(L, R, d16, d32, oc, stream, pyr::kStagesAll, count);
             return;
         }
-        // pyrDown chain: both views in one launch when their pair strides agree.
+        pyrDownChain(L, R, stream, count);
+        // Fork: the refinement levels' descriptor passes on the side stream.
+        if (side) {
+            check(hipEventRecord(evFork, stream), "hipEventRecord");
+            check(hipStreamWaitEvent(side, evFork, 0), "hipStreamWaitEvent");
+            for (int level = levels - 1; level >= 0; --level) levelCall(level, L, R, d16, d32, oc, side, pyr::kStagePrep, count);
+            check(hipEventRecord(evJoin, side), "hipEventRecord");
+        }
+        for (int level = levels; level >= 0; --level) {
+            if (side && level == levels - 1) check(hipStreamWaitEvent(stream, evJoin, 0), "hipStreamWaitEvent");
+            levelCall(level, L, R, d16, d32, oc, stream,
+                      (side && level < levels) ? pyr::kStageRest : pyr::kStagesAll, count);
+        }
+    }
+
+    // Output routing (stereo_matcher.cpp:248-253): float disparities without a
+    // confidence output still need one, so the level-0 scratch takes it.
+    BatchBuffer<float> confRoute(BatchBuffer<float> d32, BatchBuffer<float> cf) const
+    {
+        BatchBuffer<float> oc = cf;
+        if (!oc.data && d32.data) oc = {conf[0].data, conf[0].pitch, stride(conf[0], 0)};
+        return oc;
+    }
+
+    // Single-scale matcher (levels == 0): the given stages of the one level.
+    void singleCall(BatchBuffer<const uint8_t> L, BatchBuffer<const uint8_t> R,
+                    BatchBuffer<int16_t> d16, BatchBuffer<float> d32, BatchBuffer<float> oc,
+                    hipStream_t stream, int stages, int count, pyr::PathSink* sink = nullptr)
+    {
+        const int costMode = params.cost_mode == CostMode::Census ? PYRSGM_COST_CENSUS : PYRSGM_COST_BT;
+        pyr::Batch b;
+        b.n = count;
+        b.left = L.pair_stride;
+        b.right = R.pair_stride;
+        b.out16 = d16.pair_stride;
+        b.out32 = d32.pair_stride;
+        b.outc = oc.pair_stride;
+        const int ret = pyr::computeSingleScale(
+            workspaces[0], L.data, R.data, d16.data, d32.data, oc.data, L.pitch, R.pitch,
+            d16.pitch, d32.pitch, oc.pitch, width, height, params.num_disparities,
+            params.min_disparity, params.block_size, params.p1, params.p2,
+            params.uniqueness_ratio, params.disp12_max_diff, params.pre_filter_cap, costMode,
+            postFlags(0), params.num_paths, params.adaptive_p2 ? 1 : 0, params.p2_alpha,
+            params.p2_beta, params.p2_lower_mult, params.p2_upper_mult, b, stream, stages, sink);
+        if (ret != 0) throwHip("single-scale SGM", hipGetLastError());
+    }
+
+    // The pyramid images of levels 1..levels: both views in one launch per
+    // level when their pair strides agree.
+    void pyrDownChain(BatchBuffer<const uint8_t> L, BatchBuffer<const uint8_t> R,
+                      hipStream_t stream, int count)
+    {
         for (int level = 1; level <= levels; ++level) {
             const LevelInfo& parent = info[level - 1];
             const LevelInfo& li = info[level];
@@ -287,24 +322,12 @@ struct MatcherCore {
             pyr::profEnd(stream);
             if (e != hipSuccess) throwHip("pyrDown", e);
         }
-        // Fork: the refinement levels' descriptor passes on the side stream.
-        if (side) {
-            check(hipEventRecord(evFork, stream), "hipEventRecord");
-            check(hipStreamWaitEvent(side, evFork, 0), "hipStreamWaitEvent");
-            for (int level = levels - 1; level >= 0; --level) levelCall(level, L, R, d16, d32, oc, side, pyr::kStagePrep, count);
-            check(hipEventRecord(evJoin, side), "hipEventRecord");
-        }
-        for (int level = levels; level >= 0; --level) {
-            if (side && level == levels - 1) check(hipStreamWaitEvent(stream, evJoin, 0), "hipStreamWaitEvent");
-            levelCall(level, L, R, d16, d32, oc, stream,
-                      (side && level < levels) ? pyr::kStageRest : pyr::kStagesAll, count);
-        }
     }
 
     // One pyramid level (or the given stages of it) for the first `count` pairs.
     void levelCall(int level, BatchBuffer<const uint8_t> L, BatchBuffer<const uint8_t> R,
                    BatchBuffer<int16_t> d16, BatchBuffer<float> d32, BatchBuffer<float> oc,
-                   hipStream_t stream, int stages, int count)
+                   hipStream_t stream, int stages, int count, pyr::PathSink* sink = nullptr)
     {
         const int costMode = params.cost_mode == CostMode::Census ? PYRSGM_COST_CENSUS : PYRSGM_COST_BT;
         {
@@ -341,7 +364,7 @@ struct MatcherCore {
                 params.disp12_max_diff, params.pre_filter_cap, costMode, postFlags(level),
                 params.num_paths, refine ? 1 : 0, params.pyramid_min_confidence,
                 params.adaptive_p2 ? 1 : 0, params.p2_alpha, params.p2_beta, params.p2_lower_mult,
-                params.p2_upper_mult, b, stream, stages);
+                params.p2_upper_mult, b, stream, stages, sink);
             if (ret != 0) {
                 std::ostringstream oss;
                 oss << "pyramid level " << level << " (" << li.width << "x" << li.height

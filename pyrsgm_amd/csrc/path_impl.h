@@ -1039,11 +1039,64 @@ __global__ __launch_bounds__(128) void kPaths(PathArgs a)
     }
 }
 
+// Queued form: persistent workgroups take (task, block) items from a device
+// counter until the items run out; a task is one pair's level with its own
+// geometry, penalties and buffers, numbered blocks as in kPaths.  Tasks are
+// placed longest lines first by the host, so the long lines start early and
+// short ones fill the tail.
+template <int G, int K, int MODE, bool ADAPT, bool EXACT, bool NARROW, bool PK = false>
+__global__ __launch_bounds__(128) void kPathsQ(QueueArgs q)
+{
+    __shared__ int s_lut[256];
+    __shared__ int s_xchg[MODE == kRefine ? 64 * K : 1];
+    __shared__ __attribute__((aligned(16))) char s_lds[Lds<G, K, NARROW, PK>::BYTES];
+    __shared__ int s_item;
+    int cur = -1;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = (int)atomicAdd(&q.counter[0], 1u);
+        __syncthreads();
+        const int item = __builtin_amdgcn_readfirstlane(s_item);
+        if (item >= q.total) break;
+        // Task t: the last with start[t] <= item.
+        int lo = 0, hi = q.nTasks - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (q.start[mid] <= item) lo = mid;
+            else hi = mid - 1;
+        }
+        const int t = __builtin_amdgcn_readfirstlane(lo);
+        const PathArgs& a = q.tasks[t];
+        if constexpr (ADAPT) {
+            if (t != cur) {
+                for (int i = threadIdx.x; i < 256; i += 128) s_lut[i] = a.lut[i];
+                __syncthreads();
+            }
+        }
+        cur = t;
+        const int b = item - q.start[t];
+        const int nLR = a.nLRBlocks, nRL = a.nRLBlocks;
+        if (b < nLR)
+            runLine<G, K, MODE, ADAPT, EXACT, NARROW, PK, kLR>(a, 0, b, s_lut, s_xchg, s_lds);
+        else if (b < nLR + nRL)
+            runLine<G, K, MODE, ADAPT, EXACT, NARROW, PK, kRL>(a, 0, b - nLR, s_lut, s_xchg, s_lds);
+        else
+            runLine<G, K, MODE, ADAPT, EXACT, NARROW, PK, kTB>(a, 0, b - nLR - nRL, s_lut, s_xchg, s_lds);
+        // Both waves are done with the item's LDS before s_item / s_lut change.
+        __syncthreads();
+    }
+    // Every workgroup has taken its last item once all have arrived here; the
+    // last one to arrive re-arms the counters for the next launch.
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(&q.counter[1], 1u) == gridDim.x - 1) {
+            atomicExch(&q.counter[0], 0u);
+            atomicExch(&q.counter[1], 0u);
+        }
+    }
+}
+
 inline int linesPerWave(int Dp) { return Dp <= 16 ? 4 : (Dp <= 32 ? 2 : 1); }
 
-// Launches of at least this many 4-line waves use paired lines (byte volumes,
-// 16 labels, pyramid levels): four per SIMD.
-constexpr size_t kPairedWaves = 4096;
 
 template <int MODE, bool ADAPT, bool EXACT, bool NARROW>
 hipError_t dispatchPaths(const PathArgs& a, hipStream_t s)
@@ -1089,6 +1142,69 @@ hipError_t dispatchPaths(const PathArgs& a, hipStream_t s)
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+// Queued launches: the kernel of one specialisation, its occupancy and launch.
+template <int MODE, bool ADAPT, bool EXACT, bool NARROW>
+const void* queuedKernel(int Dp, int paired)
+{
+    if constexpr (NARROW && MODE != kSingle) {
+        if (paired && Dp == 16) return reinterpret_cast<const void*>(&kPathsQ<16, 1, MODE, ADAPT, EXACT, true, true>);
+    }
+    if constexpr (NARROW) {
+        switch (Dp) {
+        case 16: return reinterpret_cast<const void*>(&kPathsQ<16, 1, MODE, ADAPT, EXACT, true>);
+        case 32: return reinterpret_cast<const void*>(&kPathsQ<32, 1, MODE, ADAPT, EXACT, true>);
+        case 64: return reinterpret_cast<const void*>(&kPathsQ<64, 1, MODE, ADAPT, EXACT, true>);
+        default: return nullptr;
+        }
+    }
+    switch (Dp) {
+    case 16: return reinterpret_cast<const void*>(&kPathsQ<16, 1, MODE, ADAPT, EXACT, false>);
+    case 32: return reinterpret_cast<const void*>(&kPathsQ<32, 1, MODE, ADAPT, EXACT, false>);
+    case 64: return reinterpret_cast<const void*>(&kPathsQ<64, 1, MODE, ADAPT, EXACT, false>);
+    case 128: return reinterpret_cast<const void*>(&kPathsQ<64, 2, MODE, ADAPT, EXACT, false>);
+    case 256: return reinterpret_cast<const void*>(&kPathsQ<64, 4, MODE, ADAPT, EXACT, false>);
+    case 512: return reinterpret_cast<const void*>(&kPathsQ<64, 8, MODE, ADAPT, EXACT, false>);
+    case 1024: return reinterpret_cast<const void*>(&kPathsQ<64, 16, MODE, ADAPT, EXACT, false>);
+    default: return nullptr;
+    }
+}
+
+template <int MODE>
+const void* queuedKernelMode(int Dp, int adaptive, int exact, int narrow, int paired)
+{
+    if (MODE != kSingle && exact)
+        return adaptive ? queuedKernel<MODE, true, true, false>(Dp, 0)
+                        : queuedKernel<MODE, false, true, false>(Dp, 0);
+    if (narrow)
+        return adaptive ? queuedKernel<MODE, true, false, true>(Dp, paired)
+                        : queuedKernel<MODE, false, false, true>(Dp, paired);
+    return adaptive ? queuedKernel<MODE, true, false, false>(Dp, 0)
+                    : queuedKernel<MODE, false, false, false>(Dp, 0);
+}
+
+template <int MODE>
+int queuedOccupancy(int Dp, int adaptive, int exact, int narrow, int paired)
+{
+    const void* k = queuedKernelMode<MODE>(Dp, adaptive, exact, narrow, paired);
+    int n = 0;
+    if (!k || hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, 128, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        return 1;
+    }
+    return n > 0 ? n : 1;
+}
+
+template <int MODE>
+hipError_t launchQueuedMode(const QueueArgs& q, int Dp, int adaptive, int exact, int narrow,
+                            int paired, int grid, hipStream_t s)
+{
+    const void* k = queuedKernelMode<MODE>(Dp, adaptive, exact, narrow, paired);
+    if (!k || grid < 1) return hipErrorInvalidValue;
+    QueueArgs args = q;
+    void* params[] = {&args};
+    return hipLaunchKernel(k, dim3(grid), dim3(128), params, 0, s);
 }
 
 template <int MODE>

@@ -315,6 +315,7 @@ def bench_uneven(P, torch, log, n=64, seed=2000):
     st = server.stats()
     server.close()
     seq_pps = n / (sum(svc) * 1e-3)
+    mixed = bench_mixed(P, torch, log, reqs, dev_pairs, outs, ref, seq_pps)
     out = {"workload": f"{n} pairs, sizes 640x480..1920x1080, D 64/128/256, levels 0-3, "
                        "BT/census, median on/off", "pairs": n,
            "distinct_keys": len({tuple(sorted(k.items())) for k, _ in reqs}),
@@ -322,8 +323,40 @@ def bench_uneven(P, torch, log, n=64, seed=2000):
            "per_pair_pairs_per_s": round(seq_pps, 1),
            "speedup_vs_per_pair": round(n / span / seq_pps, 3),
            "bit_identical_to_per_pair": same,
-           "launches": st.launches, "mean_batch": round(st.mean_batch, 2)}
+           "launches": st.launches, "mean_batch": round(st.mean_batch, 2),
+           "mixed_batch": mixed}
     log(f"[bench] uneven: {out}")
+    return out
+
+
+def bench_mixed(P, torch, log, reqs, dev_pairs, outs, ref, seq_pps):
+    """The same pairs as ONE MixedBatch pass: queued path launches over all
+    pairs, per-pair stages on parallel streams, tuned, then replayed as one
+    graph.  Outputs are compared with the one-at-a-time results."""
+    specs = [(k["w"], k["h"], _params_for(P, k)) for k, _ in reqs]
+    io = [(dev_pairs[id(pair)][0], dev_pairs[id(pair)][1], outs[j], None, None)
+          for j, (k, pair) in enumerate(reqs)]
+    mb = P.MixedBatch(specs)
+    default = mb.tuning()
+    t_default = _event_ms(torch, lambda: mb.compute(io), reps=3)
+    best, tune_ms = mb.tune(io, reps=2)
+    t_eager = _event_ms(torch, lambda: mb.compute(io), reps=3)
+    mb.capture(io)
+    for o in outs:
+        o.fill_(0)
+    mb.launch()
+    torch.cuda.synchronize()
+    same = all(bool((a == b).all()) for a, b in zip(outs, ref))
+    t_graph = _event_ms(torch, mb.launch, reps=5)
+    n = len(reqs)
+    out = {"pairs_per_s": round(n / t_graph * 1000.0, 1), "ms_per_pass": round(t_graph, 3),
+           "eager_ms_per_pass": round(t_eager, 3),
+           "untuned_ms_per_pass": round(t_default, 3), "untuned_tuning": default,
+           "tuning": best, "speedup_vs_per_pair": round(n / t_graph * 1000.0 / seq_pps, 3),
+           "bit_identical_to_per_pair": same, "plan": mb.plan(),
+           "device_bytes": mb.device_memory_bytes()}
+    mb.close()
+    log(f"[bench] mixed batch: {out}")
     return out
 
 
@@ -419,6 +452,9 @@ def main() -> int:
                     help="requests per arrival rate in the serving leg")
     ap.add_argument("--only-batch", action="store_true",
                     help="run only the 32-pair batched leg and print its JSON (for profiling)")
+    ap.add_argument("--only-uneven", action="store_true",
+                    help="run only the 64-pair uneven leg (serving loop and mixed batch) and "
+                         "print its JSON (for profiling)")
     args = ap.parse_args()
     if args.derive_traffic:
         print(json.dumps(derive_traffic(*args.derive_traffic, args.calib_mib << 20), indent=1))
@@ -441,6 +477,9 @@ def main() -> int:
 
     if args.only_batch:
         print(json.dumps(bench_batched(P, torch, log)), flush=True)
+        return 0
+    if args.only_uneven:
+        print(json.dumps(bench_uneven(P, torch, log)), flush=True)
         return 0
 
     W, H, D = 1920, 1080, 256

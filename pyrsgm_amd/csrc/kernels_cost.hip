@@ -488,6 +488,75 @@ __global__ __launch_bounds__(256) void kBoxV(const int16_t* Hs, int16_t* C, int 
 }
 
 // ---- BT window cost at refinement levels (sgm_kernels.cu:1075-1110) ----------
+// Packed form of one BT descriptor: (v, v0, v1) of both channels as 16-bit
+// pairs, gradient channel low, intensity channel high.
+struct BTPacked {
+    uint32_t v, v0, v1;
+};
+
+__device__ __forceinline__ BTPacked btPack(uint2 d)
+{
+    // Output bytes: 0 = x byte k, 1 = 0, 2 = y byte k, 3 = 0.
+    constexpr uint32_t z = 0x0C000C00u;
+    return {__builtin_amdgcn_perm(d.y, d.x, z | (4u << 16) | 0u),
+            __builtin_amdgcn_perm(d.y, d.x, z | (5u << 16) | 1u),
+            __builtin_amdgcn_perm(d.y, d.x, z | (6u << 16) | 2u)};
+}
+
+// A right-view column outside the image: every channel distance is at least
+// 0x7FFF - 255, so the cost below is exactly 255 (sgm_kernels.cu:1037-1040).
+__device__ __forceinline__ BTPacked btPoison() { return {0x7FFF7FFFu, 0x7FFF7FFFu, 0x7FFF7FFFu}; }
+
+// btCost of two packed descriptors: both channels' interval distances at once
+// in signed 16-bit halves, then min(255, grad + (int >> 2)).
+__device__ __forceinline__ int btCostPacked(const BTPacked& l, const BTPacked& r)
+{
+    typedef int16_t s2 __attribute__((ext_vector_type(2)));
+    auto S = [](uint32_t x) { return __builtin_bit_cast(s2, x); };
+    const s2 zero = {0, 0};
+    const s2 c0 = __builtin_elementwise_max(__builtin_elementwise_max(S(l.v) - S(r.v1), S(r.v0) - S(l.v)), zero);
+    const s2 c1 = __builtin_elementwise_max(__builtin_elementwise_max(S(r.v) - S(l.v1), S(l.v0) - S(r.v)), zero);
+    const uint32_t c = __builtin_bit_cast(uint32_t, __builtin_elementwise_min(c0, c1));
+    return imin(255, (int)(c & 0xFFFFu) + (int)(c >> 18));
+}
+
+// Window sums of 8 consecutive labels (real disparities r0 .. r0 + 7) of a
+// pixel whose window columns x - HALF .. x + HALF all lie inside the image:
+// per window row, the 2 HALF + 1 left and 2 HALF + 8 right descriptors the
+// sums touch are loaded and unpacked once.
+template <int HALF>
+__device__ __forceinline__ void btWindowInterior(const uint2* Ld, const uint2* Rd, int W, int H,
+                                                 int x, int y, int r0, int sum[8])
+{
+    constexpr int NX = 2 * HALF + 1;
+    constexpr int NR = NX + 7;
+    const int xr0 = x - HALF - r0 - 7;  // right column of (window column 0, label 7)
+#pragma unroll 1
+    for (int dy = -HALF; dy <= HALF; ++dy) {
+        const int yy = clampi(y + dy, 0, H - 1);
+        const uint2* Lrow = Ld + (size_t)yy * W;
+        const uint2* Rrow = Rd + (size_t)yy * W;
+        uint2 lraw[NX], rraw[NR];
+#pragma unroll
+        for (int k = 0; k < NX; ++k) lraw[k] = Lrow[x - HALF + k];
+#pragma unroll
+        for (int m = 0; m < NR; ++m) rraw[m] = Rrow[clampi(xr0 + m, 0, W - 1)];
+        BTPacked lp[NX], rp[NR];
+#pragma unroll
+        for (int k = 0; k < NX; ++k) lp[k] = btPack(lraw[k]);
+#pragma unroll
+        for (int m = 0; m < NR; ++m) {
+            const int x2 = xr0 + m;
+            rp[m] = (x2 >= 0 && x2 < W) ? btPack(rraw[m]) : btPoison();
+        }
+        // Window column k, label i: right column x - HALF + k - (r0 + i) = xr0 + (7 + k - i).
+#pragma unroll
+        for (int k = 0; k < NX; ++k)
+#pragma unroll
+            for (int i = 0; i < 8; ++i) sum[i] += btCostPacked(lp[k], rp[7 + k - i]);
+    }
+}
+
 __global__ __launch_bounds__(256) void kBTLocal(const uint2* desc, RefineIn r, int32_t* labels,
                                                 const uint8_t* img, int istep, int16_t* C, int W,
                                                 int H, int D, int Dp, int minD, int maxD, int half,
@@ -522,17 +591,25 @@ __global__ __launch_bounds__(256) void kBTLocal(const uint2* desc, RefineIn r, i
         ok[i] = valid && d0 + i < D && real >= lo && real <= hi;
         sum[i] = 0;
     }
-    for (int dy = -half; dy <= half; ++dy) {
-        const int yy = clampi(y + dy, 0, H - 1);
-        const uint2* Lrow = Ld + (size_t)yy * W;
-        const uint2* Rrow = Rd + (size_t)yy * W;
-        for (int dx = -half; dx <= half; ++dx) {
-            const int xx = clampi(x + dx, 0, W - 1);
-            const uint2 li = Lrow[xx];
+    const bool interior = x - half >= 0 && x + half < W;
+    if (interior && half == 2) {
+        btWindowInterior<2>(Ld, Rd, W, H, x, y, base + d0, sum);
+    } else if (interior && half == 1) {
+        btWindowInterior<1>(Ld, Rd, W, H, x, y, base + d0, sum);
+    } else {
+        // Border columns (replicate-clamped window) and other block sizes.
+        for (int dy = -half; dy <= half; ++dy) {
+            const int yy = clampi(y + dy, 0, H - 1);
+            const uint2* Lrow = Ld + (size_t)yy * W;
+            const uint2* Rrow = Rd + (size_t)yy * W;
+            for (int dx = -half; dx <= half; ++dx) {
+                const int xx = clampi(x + dx, 0, W - 1);
+                const uint2 li = Lrow[xx];
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const int x2 = xx - (base + d0 + i);
-                sum[i] += (x2 >= 0 && x2 < W) ? btCost(li, Rrow[x2]) : 255;
+                for (int i = 0; i < 8; ++i) {
+                    const int x2 = xx - (base + d0 + i);
+                    sum[i] += (x2 >= 0 && x2 < W) ? btCost(li, Rrow[x2]) : 255;
+                }
             }
         }
     }

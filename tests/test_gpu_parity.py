@@ -184,6 +184,27 @@ def test_rejects_negative_penalties(ws):
     assert rc == -1
 
 
+@pytest.mark.parametrize("block", [1, 3, 5, 7])
+def test_bt_refine_window_sizes(ws, block):
+    """BT refinement costs: 3x3 and 5x5 windows take the packed interior form,
+    1x1 and 7x7 the generic loop; border columns always the generic loop."""
+    w, h, maxD = 203, 77, 100
+    left, right = pair(w, h, maxD, seed=block * 17 + 5)
+    pl, pr = O.pyrdown(left), O.pyrdown(right)
+    pmax = min(maxD >> 1, pl.shape[1] - 1)
+    pD = max(16, (pmax + 16) // 16 * 16)
+    parent = O.compute_level(pl, pr, pD, pmax, block=3, P1=300, P2=2400, cost=0, post=1,
+                             want32=False)
+    for D in (16, 32):
+        kw = dict(block=block, P1=300, P2=2400, cost=0, post=3, paths=3, adaptive=1,
+                  refine=True, min_conf=0.05)
+        rc, d16, d32, cf = U.level(ws, left, right, D, maxD, parent.disp16, parent.conf, **kw)
+        want = O.compute_level(left, right, D, maxD, parent.disp16, parent.conf, **kw)
+        assert rc == 0 and want.rc == 0
+        U.assert_same(f"block {block} D {D} disp16", d16, want.disp16)
+        U.assert_same(f"block {block} D {D} conf", cf, want.conf)
+
+
 def test_refine_varying_base(ws):
     """Parent disparity alternating 2 / 30 px (reference pyramid_smoke.cpp:115-134)."""
     left, right = pattern(320, 128, 16, False), pattern(320, 128, 16, True)

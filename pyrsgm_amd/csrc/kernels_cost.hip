@@ -51,7 +51,8 @@ __device__ __forceinline__ int btGrad(const uint8_t* img, int step, int W, int H
     const uint8_t* d = img + (size_t)yn * step;
     int g = ((int)r[xn] - (int)r[xp]) * 2 + ((int)u[xn] - (int)u[xp]) + ((int)d[xn] - (int)d[xp]);
     g = clampi(g, -ftzero, ftzero);
-    return g + ftzero;
+    // Stored as a byte (sgm_kernels.cu:117): wraps when ftzero exceeds 127.
+    return (int)(uint8_t)(g + ftzero);
 }
 
 // ---- descriptors + projection-key reset ----------------------------------
@@ -507,16 +508,21 @@ __device__ __forceinline__ BTPacked btPack(uint2 d)
 // 0x7FFF - 255, so the cost below is exactly 255 (sgm_kernels.cu:1037-1040).
 __device__ __forceinline__ BTPacked btPoison() { return {0x7FFF7FFFu, 0x7FFF7FFFu, 0x7FFF7FFFu}; }
 
-// btCost of two packed descriptors: both channels' interval distances at once
-// in signed 16-bit halves, then min(255, grad + (int >> 2)).
+// One BT interval distance per 16-bit half: max(0, u - v1, v0 - u) equals
+// max(u, v0) - min(u, v1) because v0 <= v1 (sgm_kernels.cu:1025-1027).
+__device__ __forceinline__ uint32_t btDistPacked(const BTPacked& l, const BTPacked& r)
+{
+    typedef uint16_t u2 __attribute__((ext_vector_type(2)));
+    auto V = [](uint32_t x) { return __builtin_bit_cast(u2, x); };
+    const u2 c0 = __builtin_elementwise_max(V(l.v), V(r.v0)) - __builtin_elementwise_min(V(l.v), V(r.v1));
+    const u2 c1 = __builtin_elementwise_max(V(r.v), V(l.v0)) - __builtin_elementwise_min(V(r.v), V(l.v1));
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_min(c0, c1));
+}
+
+// btCost of two packed descriptors: min(255, grad + (int >> 2)).
 __device__ __forceinline__ int btCostPacked(const BTPacked& l, const BTPacked& r)
 {
-    typedef int16_t s2 __attribute__((ext_vector_type(2)));
-    auto S = [](uint32_t x) { return __builtin_bit_cast(s2, x); };
-    const s2 zero = {0, 0};
-    const s2 c0 = __builtin_elementwise_max(__builtin_elementwise_max(S(l.v) - S(r.v1), S(r.v0) - S(l.v)), zero);
-    const s2 c1 = __builtin_elementwise_max(__builtin_elementwise_max(S(r.v) - S(l.v1), S(l.v0) - S(r.v)), zero);
-    const uint32_t c = __builtin_bit_cast(uint32_t, __builtin_elementwise_min(c0, c1));
+    const uint32_t c = btDistPacked(l, r);
     return imin(255, (int)(c & 0xFFFFu) + (int)(c >> 18));
 }
 
@@ -524,13 +530,21 @@ __device__ __forceinline__ int btCostPacked(const BTPacked& l, const BTPacked& r
 // pixel whose window columns x - HALF .. x + HALF all lie inside the image:
 // per window row, the 2 HALF + 1 left and 2 HALF + 8 right descriptors the
 // sums touch are loaded and unpacked once.
-template <int HALF>
+// CLAMP = false when a cost never reaches 255 (gradient values at most
+// 2 * 95: 190 + 255 / 4 < 255): the two channel terms (grad, int >> 2) are summed
+// in packed 16-bit halves (at most 25 * 255 each) and added at the end, and an
+// out-of-image right column contributes exactly (255, 0).
+template <int HALF, bool CLAMP>
 __device__ __forceinline__ void btWindowInterior(const uint2* Ld, const uint2* Rd, int W, int H,
                                                  int x, int y, int r0, int sum[8])
 {
+    typedef uint16_t u2 __attribute__((ext_vector_type(2)));
     constexpr int NX = 2 * HALF + 1;
     constexpr int NR = NX + 7;
     const int xr0 = x - HALF - r0 - 7;  // right column of (window column 0, label 7)
+    u2 acc[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) acc[i] = u2{0, 0};
 #pragma unroll 1
     for (int dy = -HALF; dy <= HALF; ++dy) {
         const int yy = clampi(y + dy, 0, H - 1);
@@ -542,21 +556,36 @@ __device__ __forceinline__ void btWindowInterior(const uint2* Ld, const uint2* R
 #pragma unroll
         for (int m = 0; m < NR; ++m) rraw[m] = Rrow[clampi(xr0 + m, 0, W - 1)];
         BTPacked lp[NX], rp[NR];
+        bool in[NR];
 #pragma unroll
         for (int k = 0; k < NX; ++k) lp[k] = btPack(lraw[k]);
 #pragma unroll
         for (int m = 0; m < NR; ++m) {
             const int x2 = xr0 + m;
-            rp[m] = (x2 >= 0 && x2 < W) ? btPack(rraw[m]) : btPoison();
+            in[m] = x2 >= 0 && x2 < W;
+            rp[m] = CLAMP && !in[m] ? btPoison() : btPack(rraw[m]);
         }
         // Window column k, label i: right column x - HALF + k - (r0 + i) = xr0 + (7 + k - i).
 #pragma unroll
         for (int k = 0; k < NX; ++k)
 #pragma unroll
-            for (int i = 0; i < 8; ++i) sum[i] += btCostPacked(lp[k], rp[7 + k - i]);
+            for (int i = 0; i < 8; ++i) {
+                const int m = 7 + k - i;
+                if constexpr (CLAMP) {
+                    sum[i] += btCostPacked(lp[k], rp[m]);
+                } else {
+                    const u2 t = __builtin_bit_cast(u2, btDistPacked(lp[k], rp[m])) >> u2{0, 2};
+                    acc[i] += in[m] ? t : u2{255, 0};
+                }
+            }
+    }
+    if constexpr (!CLAMP) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) sum[i] += (int)acc[i][0] + (int)acc[i][1];
     }
 }
 
+template <bool CLAMP>
 __global__ __launch_bounds__(256) void kBTLocal(const uint2* desc, RefineIn r, int32_t* labels,
                                                 const uint8_t* img, int istep, int16_t* C, int W,
                                                 int H, int D, int Dp, int minD, int maxD, int half,
@@ -593,9 +622,9 @@ __global__ __launch_bounds__(256) void kBTLocal(const uint2* desc, RefineIn r, i
     }
     const bool interior = x - half >= 0 && x + half < W;
     if (interior && half == 2) {
-        btWindowInterior<2>(Ld, Rd, W, H, x, y, base + d0, sum);
+        btWindowInterior<2, CLAMP>(Ld, Rd, W, H, x, y, base + d0, sum);
     } else if (interior && half == 1) {
-        btWindowInterior<1>(Ld, Rd, W, H, x, y, base + d0, sum);
+        btWindowInterior<1, CLAMP>(Ld, Rd, W, H, x, y, base + d0, sum);
     } else {
         // Border columns (replicate-clamped window) and other block sizes.
         for (int dy = -half; dy <= half; ++dy) {
@@ -726,9 +755,12 @@ hipError_t launchCost(const LevelCall& c, Workspace& ws, hipStream_t s)
             kCensusLocal<false, false><<<grid, 256, 0, s>>>(cen, r, ws.labels, c.left, c.lstep,
                                                             ws.C, c.W, c.H, c.D, c.Dp, c.minD,
                                                             c.maxD, c.batch);
+        else if (((c.cap > 15 ? c.cap : 15) | 1) > 95)  // a cost may reach 255
+            kBTLocal<true><<<grid, 256, 0, s>>>(ws.desc, r, ws.labels, c.left, c.lstep, ws.C, c.W,
+                                                c.H, c.D, c.Dp, c.minD, c.maxD, half, c.batch);
         else
-            kBTLocal<<<grid, 256, 0, s>>>(ws.desc, r, ws.labels, c.left, c.lstep, ws.C, c.W, c.H,
-                                          c.D, c.Dp, c.minD, c.maxD, half, c.batch);
+            kBTLocal<false><<<grid, 256, 0, s>>>(ws.desc, r, ws.labels, c.left, c.lstep, ws.C, c.W,
+                                                 c.H, c.D, c.Dp, c.minD, c.maxD, half, c.batch);
         return hipGetLastError();
     }
     if (c.cost == 1) {

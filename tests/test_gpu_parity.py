@@ -87,6 +87,19 @@ def test_single_scale_parity(ws, w, h, D, minD, cost, post, paths, adaptive, blo
     U.assert_same("conf", cf, want.conf)
 
 
+@pytest.mark.parametrize("cap", [95, 127, 128, 200])
+def test_single_scale_bt_pre_filter_cap(ws, cap):
+    """The stored BT gradient is a byte (sgm_kernels.cu:117): with preFilterCap
+    128 and above it wraps, and the costs follow the wrapped values."""
+    left, right = pair(150, 60, 50, seed=cap)
+    kw = dict(block=5, P1=300, P2=2400, cap=cap, cost=0, post=3, paths=3, adaptive=0)
+    rc, d16, d32, cf = U.single_scale(ws, left, right, 64, **kw)
+    want = O.compute_single_scale(left, right, 64, **kw)
+    assert rc == 0 and want.rc == 0
+    U.assert_same(f"cap {cap} disp16", d16, want.disp16)
+    U.assert_same(f"cap {cap} conf", cf, want.conf)
+
+
 def test_single_scale_output_subsets(ws):
     left, right = pair(96, 40, 40, seed=3)
     for w16, w32, wc in [(True, False, False), (False, True, True), (False, False, True),
@@ -184,20 +197,23 @@ def test_rejects_negative_penalties(ws):
     assert rc == -1
 
 
-@pytest.mark.parametrize("block", [1, 3, 5, 7])
-def test_bt_refine_window_sizes(ws, block):
-    """BT refinement costs: 3x3 and 5x5 windows take the packed interior form,
-    1x1 and 7x7 the generic loop; border columns always the generic loop."""
+@pytest.mark.parametrize("block,cap", [(1, 63), (3, 63), (5, 63), (7, 63), (5, 95), (5, 96),
+                                       (3, 200)])
+def test_bt_refine_window_sizes(ws, block, cap):
+    """BT refinement costs: 3x3 and 5x5 windows take the packed interior form
+    (packed channel sums up to preFilterCap 95, clamped per position above),
+    1x1 and 7x7 the generic loop; border columns always the generic loop.
+    preFilterCap 200 makes the stored gradient wrap as a byte."""
     w, h, maxD = 203, 77, 100
     left, right = pair(w, h, maxD, seed=block * 17 + 5)
     pl, pr = O.pyrdown(left), O.pyrdown(right)
     pmax = min(maxD >> 1, pl.shape[1] - 1)
     pD = max(16, (pmax + 16) // 16 * 16)
     parent = O.compute_level(pl, pr, pD, pmax, block=3, P1=300, P2=2400, cost=0, post=1,
-                             want32=False)
+                             want32=False, cap=cap)
     for D in (16, 32):
         kw = dict(block=block, P1=300, P2=2400, cost=0, post=3, paths=3, adaptive=1,
-                  refine=True, min_conf=0.05)
+                  refine=True, min_conf=0.05, cap=cap)
         rc, d16, d32, cf = U.level(ws, left, right, D, maxD, parent.disp16, parent.conf, **kw)
         want = O.compute_level(left, right, D, maxD, parent.disp16, parent.conf, **kw)
         assert rc == 0 and want.rc == 0

@@ -329,6 +329,69 @@ def bench_uneven(P, torch, log, n=64, seed=2000):
     return out
 
 
+def opencv_3way(left, right, D, P1, P2, threads):
+    """The reference bench's CPU path (bench/pyrsgm_bench.cpp:302-325): OpenCV
+    StereoSGBM MODE_SGBM_3WAY, timed around compute() only.  None when OpenCV
+    is not importable."""
+    try:
+        import cv2
+    except Exception as e:  # absent on these machines
+        return None, f"cv2 not importable ({type(e).__name__})"
+    import time
+    cv2.setNumThreads(threads)
+    sgbm = cv2.StereoSGBM_create(0, D, 5, P1, P2, 1, 63, 10, 0, 0, cv2.STEREO_SGBM_MODE_SGBM_3WAY)
+    sgbm.compute(left, right)
+    ts = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        out = sgbm.compute(left, right)
+        ts.append((time.perf_counter() - t0) * 1000.0)
+    return (out, sorted(ts)[len(ts) // 2]), cv2.__version__
+
+
+def bench_single_scale(P, torch, log, reps=50):
+    """640x480, D=64, BT 5x5, single scale (P1 200, P2 800, uniqueness 10, LR
+    max diff 1, 3 paths) through StereoMatcher on device-resident buffers,
+    against the oracle on the host (one thread) and, where OpenCV exists,
+    StereoSGBM MODE_SGBM_3WAY with the same parameters on every host core."""
+    import time
+    from oracle import oracle as O
+    from pyrsgm_amd import synth
+    pair = synth.config_pair("640x480_d64_bt_single")
+    W, H, D = 640, 480, 64
+    params = P.validate_params(P.Params(
+        num_disparities=D, pyramid_levels=0, cost_mode=P.CostMode.BirchfieldTomasi, p1=200,
+        p2=800, uniqueness_ratio=10, disp12_max_diff=1, block_size=5, median_filter=False))
+    L = torch.from_numpy(pair.left).cuda()
+    R = torch.from_numpy(pair.right).cuda()
+    d16 = torch.empty((H, W), dtype=torch.int16, device="cuda")
+    m = P.StereoMatcher(W, H, params)
+    t = _event_ms(torch, lambda: m.compute(L, R, d16, None, None), reps=reps)
+    got = d16.cpu().numpy()
+    m.close()
+    c0 = time.perf_counter()
+    want = O.matcher_compute(pair.left, pair.right, params, want32=False, wantc=False)
+    cpu_ms = (time.perf_counter() - c0) * 1000.0
+    out = {"workload": "640x480_d64_bt_single", "ms_per_pair": round(t, 4),
+           "pairs_per_s": round(1000.0 / t, 1),
+           "bit_identical_to_oracle": bool((got == want.disp16).all()),
+           "cpu_oracle_ms": round(cpu_ms, 1), "cpu_oracle_threads": 1}
+    cores = os.cpu_count() or 1
+    cv, info = opencv_3way(pair.left, pair.right, D, 200, 800, cores)
+    if cv is None:
+        out["opencv_sgbm_3way"] = {"available": False, "reason": info}
+    else:
+        disp, ms = cv
+        valid = (got > 0) | (disp > 0)
+        out["opencv_sgbm_3way"] = {
+            "available": True, "version": info, "threads": cores, "ms_per_pair": round(ms, 3),
+            "equal_fraction": round(float((disp == got).mean()), 6),
+            "equal_fraction_where_either_valid": round(float((disp == got)[valid].mean()), 6)
+            if valid.any() else None}
+    log(f"[bench] single scale: {out}")
+    return out
+
+
 def bench_mixed(P, torch, log, reqs, dev_pairs, outs, ref, seq_pps):
     """The same pairs as ONE MixedBatch pass: queued path launches over all
     pairs, per-pair stages on parallel streams, tuned, then replayed as one
@@ -446,6 +509,8 @@ def main() -> int:
                          "printed as JSON")
     ap.add_argument("--calib-mib", type=int, default=512)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched leg")
+    ap.add_argument("--no-single", action="store_true",
+                    help="skip the 640x480 D=64 BT single-scale leg (and its OpenCV probe)")
     ap.add_argument("--no-serve", action="store_true",
                     help="skip the uneven-batch and serving legs")
     ap.add_argument("--serve-requests", type=int, default=400,
@@ -618,6 +683,7 @@ def main() -> int:
         log(f"[bench] cpu {(c1 - c0) / reps:.2f} s/frame ({reps} runs); parity {parity}")
 
     batched = None if args.no_batch else bench_batched(P, torch, log)
+    single = None if args.no_single else bench_single_scale(P, torch, log)
     uneven = serving = None
     if not args.no_serve:
         uneven = bench_uneven(P, torch, log)
@@ -651,6 +717,7 @@ def main() -> int:
         "accuracy": {"density": round(float(valid.mean()), 4),
                      "bad1": round(float((err > 1).mean()), 4) if err.size else None},
         "batched": batched,
+        "single_scale": single,
         "uneven_batch": uneven,
         "serving": serving,
         "kernels": kernels,

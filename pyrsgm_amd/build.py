@@ -35,6 +35,7 @@ SOURCES = [
     "matcher_capi.cpp",
     "server.cpp",
     "mixed.cpp",
+    "io.cpp",
     "prof.cpp",
 ]
 

@@ -28,7 +28,7 @@ def declared_c_functions():
     """Names of the functions declared in the C headers (extern "C" blocks)."""
     names = set()
     for h in ("pyrsgm_c.h", "pyrsgm_py.h", "pyrsgm_prof.h", "pyrsgm_batch.h", "pyrsgm_serve.h",
-              "pyrsgm_mixed.h"):
+              "pyrsgm_mixed.h", "pyrsgm_io.h"):
         text = open(os.path.join(INC, h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         for m in re.finditer(r"\b(pyrsgm_[a-z0-9_]+)\s*\(", text):

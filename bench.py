@@ -284,12 +284,10 @@ def _serve_burst(P, torch, server, reqs, dev_pairs, outs):
     return res, (t1 - t0) * 1e-9
 
 
-def bench_uneven(P, torch, log, n=64, seed=2000):
-    """64 pairs drawn uniformly from {640x480, 960x540, 1280x720, 1920x1080} x
-    D {64,128,256} x pyramid_levels {0..3} x {BT, census} x median on/off, all
-    submitted at once to the serving loop (lanes of different keys overlap on
-    the device; same-key pairs share launches), against the same pairs one at a
-    time through StereoMatcher.  disp16 only, as in the reference bench."""
+def _uneven_requests(P, log, n=64, seed=2000):
+    """The uneven workload: n pairs drawn uniformly from {640x480, 960x540,
+    1280x720, 1920x1080} x D {64,128,256} x pyramid_levels {0..3} x {BT,
+    census} x median on/off."""
     import numpy as np
     rng = np.random.default_rng(seed)
     sizes = [(640, 480, 256), (960, 540, 256), (1280, 720, 256), (1920, 1080, 256)]
@@ -302,7 +300,32 @@ def bench_uneven(P, torch, log, n=64, seed=2000):
                         kinds.append({"w": w, "h": h, "D": D, "levels": lv, "cost": cost,
                                       "median": med})
     pool, dev_pairs = _pool(P, sizes, 2, seed, log)
-    reqs = _mixed_requests(rng, n, kinds, pool)
+    return _mixed_requests(rng, n, kinds, pool), dev_pairs
+
+
+def mixed_passes(P, torch, log, passes=8):
+    """Only the uneven workload as one MixedBatch (default tuning), captured
+    and replayed `passes` times (for profiling)."""
+    reqs, dev_pairs = _uneven_requests(P, log)
+    outs = [torch.empty((k["h"], k["w"]), dtype=torch.int16, device="cuda") for k, _ in reqs]
+    mb = P.MixedBatch([(k["w"], k["h"], _params_for(P, k)) for k, _ in reqs])
+    io = [(dev_pairs[id(pair)][0], dev_pairs[id(pair)][1], outs[j], None, None)
+          for j, (k, pair) in enumerate(reqs)]
+    mb.capture(io)
+    t = _event_ms(torch, mb.launch, reps=passes - 3, warm=3)
+    out = {"workload": "64 uneven pairs, one MixedBatch graph pass", "ms_per_pass": round(t, 3),
+           "passes": passes, "tuning": mb.tuning(), "plan": mb.plan()}
+    mb.close()
+    return out
+
+
+def bench_uneven(P, torch, log, n=64, seed=2000):
+    """64 pairs drawn uniformly from {640x480, 960x540, 1280x720, 1920x1080} x
+    D {64,128,256} x pyramid_levels {0..3} x {BT, census} x median on/off, all
+    submitted at once to the serving loop (lanes of different keys overlap on
+    the device; same-key pairs share launches), against the same pairs one at a
+    time through StereoMatcher.  disp16 only, as in the reference bench."""
+    reqs, dev_pairs = _uneven_requests(P, log, n, seed)
     outs = [torch.empty((k["h"], k["w"]), dtype=torch.int16, device="cuda") for k, _ in reqs]
     svc = _sequential_service_ms(P, torch, reqs, dev_pairs, outs)
     ref = [o.clone() for o in outs]
@@ -517,6 +540,9 @@ def main() -> int:
                     help="requests per arrival rate in the serving leg")
     ap.add_argument("--only-batch", action="store_true",
                     help="run only the 32-pair batched leg and print its JSON (for profiling)")
+    ap.add_argument("--only-mixed", action="store_true",
+                    help="replay only the 64-pair uneven workload as one MixedBatch graph, 3 + 5 "
+                         "passes, and print its JSON (for profiling)")
     ap.add_argument("--only-uneven", action="store_true",
                     help="run only the 64-pair uneven leg (serving loop and mixed batch) and "
                          "print its JSON (for profiling)")
@@ -545,6 +571,9 @@ def main() -> int:
         return 0
     if args.only_uneven:
         print(json.dumps(bench_uneven(P, torch, log)), flush=True)
+        return 0
+    if args.only_mixed:
+        print(json.dumps(mixed_passes(P, torch, log)), flush=True)
         return 0
 
     W, H, D = 1920, 1080, 256

@@ -10,6 +10,8 @@
 // Volumes are int16 [y][x][Dp], labels innermost, stored as SHRT_MAX - cost;
 // every thread produces 8 consecutive labels of one pixel and writes them with
 // one 16-byte store.
+#include <climits>
+
 #include "device_common.h"
 #include "internal.h"
 
@@ -648,6 +650,180 @@ __global__ __launch_bounds__(256) void kBTLocal(const uint2* desc, RefineIn r, i
     storeCost8(C + p * Dp + d0, v);
 }
 
+// Tiled form for 16-label windows (every StereoMatcher refinement level) when
+// no cost can reach 255 (packed channel sums, see btWindowInterior) and the
+// window is at most 7x7.  Block: one row segment of kTilePx pixels.  The
+// window sum of pixel x at real disparity r is the sum over the 2 HALF + 1
+// window columns xx of the column sum V[r][xx] = sum over the window rows of
+// cost(L(xx, yy), R(xx - r, yy)) (columns and rows replicate-clamped, 255 where
+// xx - r leaves the image).  V depends on (r, xx) only, so the block computes
+// it once for every real in the union [rlo, rhi] of its pixels' windows and
+// every window column, in LDS, and each (pixel, label) then adds 2 HALF + 1 of
+// them.  A segment whose union is wider than kTileR reals (a depth edge with
+// far-apart windows) takes the direct per-pixel form instead.
+constexpr int kTilePx = 64;
+constexpr int kTileR = 48;
+
+template <int HALF>
+__global__ __launch_bounds__(256) void kBTLocalTile(const uint2* desc, RefineIn r, int32_t* labels,
+                                                    const uint8_t* img, int istep, int16_t* C,
+                                                    int W, int H, int minD, int maxD, Batch b)
+{
+    typedef uint16_t u2 __attribute__((ext_vector_type(2)));
+    constexpr int NC = kTilePx + 2 * HALF;  // window columns of the segment
+    constexpr int NY = 2 * HALF + 1;
+    constexpr int NR2 = NC + kTileR - 1;    // right columns the column sums touch
+    constexpr int D = 16;
+    __shared__ BTPacked s_l[NY][NC];
+    __shared__ BTPacked s_r[NY][NR2];
+    __shared__ uint32_t s_v[kTileR][NC];
+    __shared__ int s_base[kTilePx];
+    __shared__ int s_lo, s_hi;
+    const size_t pr = blockIdx.z;
+    desc = pairAt(desc, pr, b.ws);
+    labels = pairAt(labels, pr, b.ws);
+    C = pairAt(C, pr, b.ws);
+    img = pairAt(img, pr, b.left);
+    r.pdisp = pairAt(r.pdisp, pr, b.pdisp);
+    r.pconf = pairAt(r.pconf, pr, b.pconf);
+    const int y = blockIdx.y, x0 = blockIdx.x * kTilePx, tid = threadIdx.x;
+    const uint2* Ld = desc;
+    const uint2* Rd = desc + (size_t)W * H;
+    static_assert(kTilePx == 64, "the first wave holds the segment's pixels");
+    if (tid < kTilePx) {
+        const int x = x0 + tid;
+        int base = 0, valid = 0;
+        if (x < W) {
+            refineLabel(r, x, y, D, minD, maxD, base, valid);
+            labels[(size_t)y * W + x] = labelWord(base, valid, img[(size_t)y * istep + x]);
+        }
+        s_base[tid] = valid ? base : INT_MIN;
+        int lo = valid ? base : INT_MAX, hi = valid ? base + D - 1 : INT_MIN;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            lo = imin(lo, __shfl_xor(lo, off));
+            hi = imax(hi, __shfl_xor(hi, off));
+        }
+        if (tid == 0) {
+            s_lo = lo;
+            s_hi = hi;
+        }
+    }
+    __syncthreads();
+    const int rlo = s_lo, rhi = s_hi;
+    if (rlo > rhi) {
+        // No valid pixel: every label is SHRT_MAX.
+        const int px = tid >> 2, q = (tid & 3) * 4;
+        const int x = x0 + px;
+        if (x < W) {
+            uint2 o;
+            o.x = 0u;
+            o.y = 0u;
+            *reinterpret_cast<uint2*>(C + ((size_t)y * W + x) * D + q) = o;
+        }
+        return;
+    }
+    if (rhi - rlo + 1 <= kTileR) {
+        const int xlo = clampi(x0 - HALF, 0, W - 1);
+        const int xl2 = xlo - rhi;  // right column of s_r[.][0]
+        const int nr2 = clampi(x0 + kTilePx - 1 + HALF, 0, W - 1) - rlo - xl2 + 1;  // <= NR2
+        for (int i = tid; i < NY * NC; i += 256) {
+            const int dy = i / NC, c = i - dy * NC;
+            const int yy = clampi(y + dy - HALF, 0, H - 1);
+            s_l[dy][c] = btPack(Ld[(size_t)yy * W + clampi(x0 - HALF + c, 0, W - 1)]);
+        }
+        for (int i = tid; i < NY * NR2; i += 256) {
+            const int dy = i / NR2, j = i - dy * NR2;
+            if (j >= nr2) continue;
+            const int yy = clampi(y + dy - HALF, 0, H - 1);
+            const int x2 = xl2 + j;
+            s_r[dy][j] = (x2 >= 0 && x2 < W) ? btPack(Rd[(size_t)yy * W + x2]) : btPoison();
+        }
+        __syncthreads();
+        const int R = rhi - rlo + 1;
+#pragma unroll 1
+        for (int i = tid; i < R * NC; i += 256) {
+            const int rr = i / NC, c = i - rr * NC;
+            const int x2 = clampi(x0 - HALF + c, 0, W - 1) - (rlo + rr);
+            u2 acc = u2{(uint16_t)(255 * NY), 0};
+            if (x2 >= 0 && x2 < W) {
+                acc = u2{0, 0};
+                const int j = x2 - xl2;
+#pragma unroll
+                for (int dy = 0; dy < NY; ++dy)
+                    acc += __builtin_bit_cast(u2, btDistPacked(s_l[dy][c], s_r[dy][j])) >> u2{0, 2};
+            }
+            s_v[rr][c] = __builtin_bit_cast(uint32_t, acc);
+        }
+        __syncthreads();
+        // Outputs: four threads per pixel, four labels each.
+        const int px = tid >> 2, q = (tid & 3) * 4;
+        const int x = x0 + px;
+        if (x >= W) return;
+        const int base = s_base[px];
+        const int lo = imax(minD, x - W + 1), hi = imin(maxD, x);
+        int v[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int real = base + q + i;
+            const bool ok = base != INT_MIN && real >= lo && real <= hi;
+            int sum = kMax;
+            if (ok) {
+                const int rr = real - rlo;
+                u2 acc = u2{0, 0};
+#pragma unroll
+                for (int k = 0; k < NY; ++k) acc += __builtin_bit_cast(u2, s_v[rr][px + k]);
+                sum = imin((int)acc[0] + (int)acc[1], kMax - 1);
+            }
+            v[i] = kMax - sum;
+        }
+        uint2 o;
+        o.x = (uint32_t)(uint16_t)v[0] | ((uint32_t)(uint16_t)v[1] << 16);
+        o.y = (uint32_t)(uint16_t)v[2] | ((uint32_t)(uint16_t)v[3] << 16);
+        *reinterpret_cast<uint2*>(C + ((size_t)y * W + x) * D + q) = o;
+        return;
+    }
+    // Wide union (about 0.5% of segments in the bench scenes): the plain
+    // per-pixel loop, two threads per pixel, eight labels each.
+    if (tid >= 2 * kTilePx) return;
+    const int px = tid >> 1, d0 = (tid & 1) * 8;
+    const int x = x0 + px;
+    if (x >= W) return;
+    const int base = s_base[px];
+    const bool valid = base != INT_MIN;
+    const int lo = imax(minD, x - W + 1), hi = imin(maxD, x);
+    int sum[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) sum[i] = 0;
+    if (valid) {
+        {
+#pragma unroll 1
+            for (int dy = -HALF; dy <= HALF; ++dy) {
+                const int yy = clampi(y + dy, 0, H - 1);
+                const uint2* Lrow = Ld + (size_t)yy * W;
+                const uint2* Rrow = Rd + (size_t)yy * W;
+#pragma unroll 1
+                for (int dx = -HALF; dx <= HALF; ++dx) {
+                    const int xx = clampi(x + dx, 0, W - 1);
+                    const uint2 li = Lrow[xx];
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        const int x2 = xx - (base + d0 + i);
+                        sum[i] += (x2 >= 0 && x2 < W) ? btCost(li, Rrow[x2]) : 255;
+                    }
+                }
+            }
+        }
+    }
+    int v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int real = base + d0 + i;
+        v[i] = valid && real >= lo && real <= hi ? imin(sum[i], kMax - 1) : kMax;
+    }
+    storeCost8(C + ((size_t)y * W + x) * D + d0, v);
+}
+
 // ---- pyrDown (pyrdown.cu:16-52) ------------------------------------------------
 __device__ __forceinline__ int reflect101(int i, int n)
 {
@@ -755,7 +931,15 @@ hipError_t launchCost(const LevelCall& c, Workspace& ws, hipStream_t s)
             kCensusLocal<false, false><<<grid, 256, 0, s>>>(cen, r, ws.labels, c.left, c.lstep,
                                                             ws.C, c.W, c.H, c.D, c.Dp, c.minD,
                                                             c.maxD, c.batch);
-        else if (((c.cap > 15 ? c.cap : 15) | 1) > 95)  // a cost may reach 255
+        else if (((c.cap > 15 ? c.cap : 15) | 1) <= 95 && c.Dp == 16 && c.D == 16 && half <= 3) {
+            const dim3 tg((c.W + kTilePx - 1) / kTilePx, c.H, c.batch.n);
+            switch (half) {
+            case 0: kBTLocalTile<0><<<tg, 256, 0, s>>>(ws.desc, r, ws.labels, c.left, c.lstep, ws.C, c.W, c.H, c.minD, c.maxD, c.batch); break;
+            case 1: kBTLocalTile<1><<<tg, 256, 0, s>>>(ws.desc, r, ws.labels, c.left, c.lstep, ws.C, c.W, c.H, c.minD, c.maxD, c.batch); break;
+            case 2: kBTLocalTile<2><<<tg, 256, 0, s>>>(ws.desc, r, ws.labels, c.left, c.lstep, ws.C, c.W, c.H, c.minD, c.maxD, c.batch); break;
+            default: kBTLocalTile<3><<<tg, 256, 0, s>>>(ws.desc, r, ws.labels, c.left, c.lstep, ws.C, c.W, c.H, c.minD, c.maxD, c.batch); break;
+            }
+        } else if (((c.cap > 15 ? c.cap : 15) | 1) > 95)  // a cost may reach 255
             kBTLocal<true><<<grid, 256, 0, s>>>(ws.desc, r, ws.labels, c.left, c.lstep, ws.C, c.W,
                                                 c.H, c.D, c.Dp, c.minD, c.maxD, half, c.batch);
         else

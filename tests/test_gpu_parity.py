@@ -221,6 +221,25 @@ def test_bt_refine_window_sizes(ws, block, cap):
         U.assert_same(f"block {block} D {D} conf", cf, want.conf)
 
 
+def test_bt_refine_sparse_parent(ws):
+    """BT refinement where whole 64-pixel row segments have no valid parent
+    (every label SHRT_MAX), next to segments with a steep disparity ramp."""
+    w, h = 300, 40
+    left, right = pair(w, h, 120, seed=99)
+    pw, ph = (w + 1) // 2, (h + 1) // 2
+    xs = np.arange(pw)[None, :]
+    pdisp = np.where((xs // 40) % 2 == 0, -16, (xs * 16 * 3) // 8).astype(np.int16)
+    pdisp = np.repeat(pdisp, ph, axis=0)
+    pconf = np.full((ph, pw), 0.5, np.float32)
+    kw = dict(block=5, P1=300, P2=2400, cost=0, post=3, paths=3, refine=True, min_conf=0.05,
+              adaptive=1)
+    rc, d16, d32, cf = U.level(ws, left, right, 16, 150, pdisp, pconf, **kw)
+    want = O.compute_level(left, right, 16, 150, pdisp, pconf, **kw)
+    assert rc == 0 and want.rc == 0
+    U.assert_same("disp16", d16, want.disp16)
+    U.assert_same("conf", cf, want.conf)
+
+
 def test_refine_varying_base(ws):
     """Parent disparity alternating 2 / 30 px (reference pyramid_smoke.cpp:115-134)."""
     left, right = pattern(320, 128, 16, False), pattern(320, 128, 16, True)

@@ -303,6 +303,23 @@ def _uneven_requests(P, log, n=64, seed=2000):
     return _mixed_requests(rng, n, kinds, pool), dev_pairs
 
 
+def profile_pair(P, torch, spec, reps=10):
+    """One synthetic pair of the given size, D, pyramid levels and cost (bt |
+    census; the bench's penalties, median on, disp16 out) through StereoMatcher."""
+    from pyrsgm_amd import synth
+    size, D, levels, cost = spec.split(",")
+    W, H = (int(v) for v in size.split("x"))
+    k = {"w": W, "h": H, "D": int(D), "levels": int(levels), "cost": cost, "median": True}
+    pair = synth.make_pair(W, H, int(D) - 1, seed=5)
+    L = torch.from_numpy(pair.left).cuda()
+    R = torch.from_numpy(pair.right).cuda()
+    d16 = torch.empty((H, W), dtype=torch.int16, device="cuda")
+    m = P.StereoMatcher(W, H, _params_for(P, k))
+    t = _event_ms(torch, lambda: m.compute(L, R, d16, None, None), reps=reps)
+    m.close()
+    return {"pair": spec, "ms_per_pair": round(t, 4), "reps": reps}
+
+
 def mixed_passes(P, torch, log, passes=8):
     """Only the uneven workload as one MixedBatch (default tuning), captured
     and replayed `passes` times (for profiling)."""
@@ -543,6 +560,9 @@ def main() -> int:
     ap.add_argument("--only-mixed", action="store_true",
                     help="replay only the 64-pair uneven workload as one MixedBatch graph, 3 + 5 "
                          "passes, and print its JSON (for profiling)")
+    ap.add_argument("--profile-pair", metavar="WxH,D,LEVELS,COST",
+                    help="run one synthetic pair (e.g. 1920x1080,256,2,bt) 10 times through "
+                         "StereoMatcher and print its mean device time (for profiling)")
     ap.add_argument("--only-uneven", action="store_true",
                     help="run only the 64-pair uneven leg (serving loop and mixed batch) and "
                          "print its JSON (for profiling)")
@@ -574,6 +594,9 @@ def main() -> int:
         return 0
     if args.only_mixed:
         print(json.dumps(mixed_passes(P, torch, log)), flush=True)
+        return 0
+    if args.profile_pair:
+        print(json.dumps(profile_pair(P, torch, args.profile_pair)), flush=True)
         return 0
 
     W, H, D = 1920, 1080, 256

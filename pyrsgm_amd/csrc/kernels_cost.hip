@@ -652,16 +652,18 @@ __global__ __launch_bounds__(256) void kBTLocal(const uint2* desc, RefineIn r, i
 
 // Tiled form for 16-label windows (every StereoMatcher refinement level) when
 // no cost can reach 255 (packed channel sums, see btWindowInterior) and the
-// window is at most 7x7.  Block: one row segment of kTilePx pixels.  The
-// window sum of pixel x at real disparity r is the sum over the 2 HALF + 1
-// window columns xx of the column sum V[r][xx] = sum over the window rows of
-// cost(L(xx, yy), R(xx - r, yy)) (columns and rows replicate-clamped, 255 where
-// xx - r leaves the image).  V depends on (r, xx) only, so the block computes
-// it once for every real in the union [rlo, rhi] of its pixels' windows and
-// every window column, in LDS, and each (pixel, label) then adds 2 HALF + 1 of
-// them.  A segment whose union is wider than kTileR reals (a depth edge with
-// far-apart windows) takes the direct per-pixel form instead.
+// window is at most 7x7.  Block: kTileRows rows of kTilePx pixels.  The window
+// sum of pixel (x, y) at real disparity r is the sum over its 2 HALF + 1
+// window columns xx of the column sum V_y[r][xx] = sum over rows
+// y - HALF .. y + HALF of cost(L(xx, yy), R(xx - r, yy)) (rows and columns
+// replicate-clamped, 255 where xx - r leaves the image).  V depends on
+// (y, r, xx) only: the block computes it for every real in the union
+// [rlo, rhi] of its pixels' windows and every window column, in LDS, sliding
+// it down one row at a time (one cost enters, one leaves), and each
+// (pixel, label) adds 2 HALF + 1 entries.  A block whose union is wider than
+// kTileR reals (a depth edge with far-apart windows) takes the plain loop.
 constexpr int kTilePx = 64;
+constexpr int kTileRows = 4;
 constexpr int kTileR = 48;
 
 template <int HALF>
@@ -670,15 +672,17 @@ __global__ __launch_bounds__(256) void kBTLocalTile(const uint2* desc, RefineIn 
                                                     int W, int H, int minD, int maxD, Batch b)
 {
     typedef uint16_t u2 __attribute__((ext_vector_type(2)));
-    constexpr int NC = kTilePx + 2 * HALF;  // window columns of the segment
-    constexpr int NY = 2 * HALF + 1;
-    constexpr int NR2 = NC + kTileR - 1;    // right columns the column sums touch
+    constexpr int NC = kTilePx + 2 * HALF;       // window columns of the block
+    constexpr int NY = 2 * HALF + 1;             // window rows of one output row
+    constexpr int NYT = kTileRows + 2 * HALF;    // window rows of the block
+    constexpr int NR2 = NC + kTileR - 1;         // right columns the column sums touch
     constexpr int D = 16;
-    __shared__ BTPacked s_l[NY][NC];
-    __shared__ BTPacked s_r[NY][NR2];
+    static_assert(kTilePx == 64 && kTileRows * kTilePx == 256, "one wave per block row");
+    __shared__ BTPacked s_l[NYT][NC];
+    __shared__ BTPacked s_r[NYT][NR2];
     __shared__ uint32_t s_v[kTileR][NC];
-    __shared__ int s_base[kTilePx];
-    __shared__ int s_lo, s_hi;
+    __shared__ int s_base[kTileRows][kTilePx];
+    __shared__ int s_lo[kTileRows], s_hi[kTileRows];
     const size_t pr = blockIdx.z;
     desc = pairAt(desc, pr, b.ws);
     labels = pairAt(labels, pr, b.ws);
@@ -686,117 +690,135 @@ __global__ __launch_bounds__(256) void kBTLocalTile(const uint2* desc, RefineIn 
     img = pairAt(img, pr, b.left);
     r.pdisp = pairAt(r.pdisp, pr, b.pdisp);
     r.pconf = pairAt(r.pconf, pr, b.pconf);
-    const int y = blockIdx.y, x0 = blockIdx.x * kTilePx, tid = threadIdx.x;
+    const int y0 = blockIdx.y * kTileRows, x0 = blockIdx.x * kTilePx, tid = threadIdx.x;
     const uint2* Ld = desc;
     const uint2* Rd = desc + (size_t)W * H;
-    static_assert(kTilePx == 64, "the first wave holds the segment's pixels");
-    if (tid < kTilePx) {
-        const int x = x0 + tid;
+    // Label windows: wave w holds row y0 + w.
+    {
+        const int row = tid >> 6, px = tid & 63;
+        const int x = x0 + px, y = y0 + row;
         int base = 0, valid = 0;
-        if (x < W) {
+        if (x < W && y < H) {
             refineLabel(r, x, y, D, minD, maxD, base, valid);
             labels[(size_t)y * W + x] = labelWord(base, valid, img[(size_t)y * istep + x]);
         }
-        s_base[tid] = valid ? base : INT_MIN;
+        s_base[row][px] = valid ? base : INT_MIN;
         int lo = valid ? base : INT_MAX, hi = valid ? base + D - 1 : INT_MIN;
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
             lo = imin(lo, __shfl_xor(lo, off));
             hi = imax(hi, __shfl_xor(hi, off));
         }
-        if (tid == 0) {
-            s_lo = lo;
-            s_hi = hi;
+        if (px == 0) {
+            s_lo[row] = lo;
+            s_hi[row] = hi;
         }
     }
     __syncthreads();
-    const int rlo = s_lo, rhi = s_hi;
+    int rlo = s_lo[0], rhi = s_hi[0];
+#pragma unroll
+    for (int k = 1; k < kTileRows; ++k) {
+        rlo = imin(rlo, s_lo[k]);
+        rhi = imax(rhi, s_hi[k]);
+    }
+    // Output of thread (px, q): labels q .. q + 3 of pixel px of one row.
+    const int opx = tid >> 2, q = (tid & 3) * 4;
+    const int ox = x0 + opx;
+    auto store4 = [&](int y, const int (&v)[4]) {
+        uint2 o;
+        o.x = (uint32_t)(uint16_t)(kMax - v[0]) | ((uint32_t)(uint16_t)(kMax - v[1]) << 16);
+        o.y = (uint32_t)(uint16_t)(kMax - v[2]) | ((uint32_t)(uint16_t)(kMax - v[3]) << 16);
+        *reinterpret_cast<uint2*>(C + ((size_t)y * W + ox) * D + q) = o;
+    };
     if (rlo > rhi) {
         // No valid pixel: every label is SHRT_MAX.
-        const int px = tid >> 2, q = (tid & 3) * 4;
-        const int x = x0 + px;
-        if (x < W) {
-            uint2 o;
-            o.x = 0u;
-            o.y = 0u;
-            *reinterpret_cast<uint2*>(C + ((size_t)y * W + x) * D + q) = o;
-        }
+        const int v[4] = {kMax, kMax, kMax, kMax};
+        if (ox < W)
+            for (int row = 0; row < kTileRows && y0 + row < H; ++row) store4(y0 + row, v);
         return;
     }
     if (rhi - rlo + 1 <= kTileR) {
         const int xlo = clampi(x0 - HALF, 0, W - 1);
         const int xl2 = xlo - rhi;  // right column of s_r[.][0]
         const int nr2 = clampi(x0 + kTilePx - 1 + HALF, 0, W - 1) - rlo - xl2 + 1;  // <= NR2
-        for (int i = tid; i < NY * NC; i += 256) {
+        for (int i = tid; i < NYT * NC; i += 256) {
             const int dy = i / NC, c = i - dy * NC;
-            const int yy = clampi(y + dy - HALF, 0, H - 1);
+            const int yy = clampi(y0 - HALF + dy, 0, H - 1);
             s_l[dy][c] = btPack(Ld[(size_t)yy * W + clampi(x0 - HALF + c, 0, W - 1)]);
         }
-        for (int i = tid; i < NY * NR2; i += 256) {
+        for (int i = tid; i < NYT * NR2; i += 256) {
             const int dy = i / NR2, j = i - dy * NR2;
             if (j >= nr2) continue;
-            const int yy = clampi(y + dy - HALF, 0, H - 1);
+            const int yy = clampi(y0 - HALF + dy, 0, H - 1);
             const int x2 = xl2 + j;
             s_r[dy][j] = (x2 >= 0 && x2 < W) ? btPack(Rd[(size_t)yy * W + x2]) : btPoison();
         }
         __syncthreads();
-        const int R = rhi - rlo + 1;
+        const int RN = (rhi - rlo + 1) * NC;
+        auto ev = [&](int dy, int c, int jr) {
+            return __builtin_bit_cast(u2, btDistPacked(s_l[dy][c], s_r[dy][jr])) >> u2{0, 2};
+        };
 #pragma unroll 1
-        for (int i = tid; i < R * NC; i += 256) {
-            const int rr = i / NC, c = i - rr * NC;
-            const int x2 = clampi(x0 - HALF + c, 0, W - 1) - (rlo + rr);
-            u2 acc = u2{(uint16_t)(255 * NY), 0};
-            if (x2 >= 0 && x2 < W) {
-                acc = u2{0, 0};
-                const int j = x2 - xl2;
+        for (int row = 0; row < kTileRows; ++row) {
+            // Entry (rr, c) belongs to thread i % 256 for every row: it slides its
+            // own column sums down in place.
+#pragma unroll 1
+            for (int i = tid; i < RN; i += 256) {
+                const int rr = i / NC, c = i - rr * NC;
+                const int x2 = clampi(x0 - HALF + c, 0, W - 1) - (rlo + rr);
+                if (x2 >= 0 && x2 < W) {
+                    const int jr = x2 - xl2;
+                    u2 acc;
+                    if (row == 0) {
+                        acc = u2{0, 0};
 #pragma unroll
-                for (int dy = 0; dy < NY; ++dy)
-                    acc += __builtin_bit_cast(u2, btDistPacked(s_l[dy][c], s_r[dy][j])) >> u2{0, 2};
+                        for (int dy = 0; dy < NY; ++dy) acc += ev(dy, c, jr);
+                    } else {
+                        acc = __builtin_bit_cast(u2, s_v[rr][c]) + ev(row + NY - 1, c, jr) - ev(row - 1, c, jr);
+                    }
+                    s_v[rr][c] = __builtin_bit_cast(uint32_t, acc);
+                } else if (row == 0) {
+                    s_v[rr][c] = 255u * NY;  // (255 * NY, 0): every window row outside the image
+                }
             }
-            s_v[rr][c] = __builtin_bit_cast(uint32_t, acc);
-        }
-        __syncthreads();
-        // Outputs: four threads per pixel, four labels each.
-        const int px = tid >> 2, q = (tid & 3) * 4;
-        const int x = x0 + px;
-        if (x >= W) return;
-        const int base = s_base[px];
-        const int lo = imax(minD, x - W + 1), hi = imin(maxD, x);
-        int v[4];
+            __syncthreads();
+            const int y = y0 + row;
+            if (y < H && ox < W) {
+                const int base = s_base[row][opx];
+                const int lo = imax(minD, ox - W + 1), hi = imin(maxD, ox);
+                int v[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int real = base + q + i;
-            const bool ok = base != INT_MIN && real >= lo && real <= hi;
-            int sum = kMax;
-            if (ok) {
-                const int rr = real - rlo;
-                u2 acc = u2{0, 0};
+                for (int i = 0; i < 4; ++i) {
+                    const int real = base + q + i;
+                    v[i] = kMax;
+                    if (base != INT_MIN && real >= lo && real <= hi) {
+                        const int rr = real - rlo;
+                        u2 sum = u2{0, 0};
 #pragma unroll
-                for (int k = 0; k < NY; ++k) acc += __builtin_bit_cast(u2, s_v[rr][px + k]);
-                sum = imin((int)acc[0] + (int)acc[1], kMax - 1);
+                        for (int k = 0; k < NY; ++k) sum += __builtin_bit_cast(u2, s_v[rr][opx + k]);
+                        v[i] = imin((int)sum[0] + (int)sum[1], kMax - 1);
+                    }
+                }
+                store4(y, v);
             }
-            v[i] = kMax - sum;
+            __syncthreads();
         }
-        uint2 o;
-        o.x = (uint32_t)(uint16_t)v[0] | ((uint32_t)(uint16_t)v[1] << 16);
-        o.y = (uint32_t)(uint16_t)v[2] | ((uint32_t)(uint16_t)v[3] << 16);
-        *reinterpret_cast<uint2*>(C + ((size_t)y * W + x) * D + q) = o;
         return;
     }
-    // Wide union (about 0.5% of segments in the bench scenes): the plain
-    // per-pixel loop, two threads per pixel, eight labels each.
-    if (tid >= 2 * kTilePx) return;
-    const int px = tid >> 1, d0 = (tid & 1) * 8;
-    const int x = x0 + px;
-    if (x >= W) return;
-    const int base = s_base[px];
+    // Wide union (about 0.5% of blocks in the bench scenes): the plain
+    // per-pixel loop, one thread per pixel, four labels at a time.
+    const int prow = tid >> 6, ppx = tid & 63;
+    const int x = x0 + ppx, y = y0 + prow;
+    if (x >= W || y >= H) return;
+    const int base = s_base[prow][ppx];
     const bool valid = base != INT_MIN;
     const int lo = imax(minD, x - W + 1), hi = imin(maxD, x);
-    int sum[8];
+#pragma unroll 1
+    for (int d0 = 0; d0 < D; d0 += 8) {
+        int sum[8];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) sum[i] = 0;
-    if (valid) {
-        {
+        for (int i = 0; i < 8; ++i) sum[i] = 0;
+        if (valid) {
 #pragma unroll 1
             for (int dy = -HALF; dy <= HALF; ++dy) {
                 const int yy = clampi(y + dy, 0, H - 1);
@@ -814,14 +836,14 @@ __global__ __launch_bounds__(256) void kBTLocalTile(const uint2* desc, RefineIn 
                 }
             }
         }
-    }
-    int v[8];
+        int v[8];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int real = base + d0 + i;
-        v[i] = valid && real >= lo && real <= hi ? imin(sum[i], kMax - 1) : kMax;
+        for (int i = 0; i < 8; ++i) {
+            const int real = base + d0 + i;
+            v[i] = valid && real >= lo && real <= hi ? imin(sum[i], kMax - 1) : kMax;
+        }
+        storeCost8(C + ((size_t)y * W + x) * D + d0, v);
     }
-    storeCost8(C + ((size_t)y * W + x) * D + d0, v);
 }
 
 // ---- pyrDown (pyrdown.cu:16-52) ------------------------------------------------
@@ -932,7 +954,7 @@ hipError_t launchCost(const LevelCall& c, Workspace& ws, hipStream_t s)
                                                             ws.C, c.W, c.H, c.D, c.Dp, c.minD,
                                                             c.maxD, c.batch);
         else if (((c.cap > 15 ? c.cap : 15) | 1) <= 95 && c.Dp == 16 && c.D == 16 && half <= 3) {
-            const dim3 tg((c.W + kTilePx - 1) / kTilePx, c.H, c.batch.n);
+            const dim3 tg((c.W + kTilePx - 1) / kTilePx, (c.H + kTileRows - 1) / kTileRows, c.batch.n);
             switch (half) {
             case 0: kBTLocalTile<0><<<tg, 256, 0, s>>>(ws.desc, r, ws.labels, c.left, c.lstep, ws.C, c.W, c.H, c.minD, c.maxD, c.batch); break;
             case 1: kBTLocalTile<1><<<tg, 256, 0, s>>>(ws.desc, r, ws.labels, c.left, c.lstep, ws.C, c.W, c.H, c.minD, c.maxD, c.batch); break;

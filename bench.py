@@ -160,6 +160,8 @@ def bench_batched(P, torch, log, n=32, W=960, H=540, D=128):
     m.close()
     d16.fill_(0)
     bm = P.BatchMatcher(W, H, n, params)
+    t_untuned = _event_ms(torch, lambda: bm.compute(L, R, d16, d32, cf), reps=10)
+    tuning, _ = bm.tune(L, R, d16, d32, cf, reps=5)
     t_eager = _event_ms(torch, lambda: bm.compute(L, R, d16, d32, cf), reps=10)
     # Batched roofline of the dominant launch (paths at 960x540x16 for all pairs):
     # HIP events around that launch in eager batches.
@@ -188,6 +190,7 @@ def bench_batched(P, torch, log, n=32, W=960, H=540, D=128):
     out = {"workload": f"{n}x{W}x{H}_d{D}_census_l2", "pairs": n,
            "pairs_per_s": round(n / t_graph * 1000.0, 1),
            "ms_per_batch": round(t_graph, 4), "eager_ms_per_batch": round(t_eager, 4),
+           "untuned_eager_ms_per_batch": round(t_untuned, 4), "tuning": tuning,
            "per_pair_pairs_per_s": round(n / t_pp * 1000.0, 1),
            "per_pair_ms": round(t_pp / n, 4),
            "speedup_vs_per_pair": round(t_pp / t_graph, 3),

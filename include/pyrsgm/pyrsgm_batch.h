@@ -37,6 +37,14 @@ struct BatchBuffer {
     size_t pair_stride = 0;
 };
 
+/// Launch shapes of a BatchMatcher; every choice gives the same bits.
+struct BatchTuning {
+    int census_rows = 4;   ///< rows per block of the refinement census cost: 2, 4 or 8
+    int paired = -1;       ///< paired lines for 16-label byte volumes: -1 by launch size, 0 off, 1 on
+    int xcd_remap = -1;    ///< XCD-contiguous T->B column groups: -1 default, 0 off, 1 on
+    int side_stream = 1;   ///< refinement descriptor passes on a side stream (batches of 2+)
+};
+
 class BatchMatcher {
 public:
     BatchMatcher(int width, int height, int batch, const Params& params);
@@ -57,6 +65,16 @@ public:
                  int count = -1);
     /// Replay the captured graph on `stream`.  Throws if nothing was captured.
     void launch(hipStream_t stream = nullptr);
+
+    void setTuning(const BatchTuning& t);  ///< drops a captured graph
+    BatchTuning tuning() const;
+    /// Time candidate launch shapes on these buffers (reps passes each after a
+    /// warm-up, coordinate descent), keep and return the fastest; drops a
+    /// captured graph.  The choice is remembered per (size, batch, Params) for
+    /// the life of the process and used by later matchers of that key.
+    BatchTuning tune(BatchBuffer<const uint8_t> left, BatchBuffer<const uint8_t> right,
+                     BatchBuffer<int16_t> disp16, BatchBuffer<float> disp32,
+                     BatchBuffer<float> conf, hipStream_t stream, int reps, double* best_ms);
 
     int batch() const;
     int width() const;
@@ -98,6 +116,17 @@ int pyrsgm_batch_capture(void* matcher,
                          float* d_conf, size_t conf_pitch, size_t conf_stride,
                          int count, char* err, int errlen);
 int pyrsgm_batch_launch(void* matcher, hipStream_t stream, char* err, int errlen);
+/* BatchTuning as {census_rows, paired, xcd_remap, side_stream}. */
+int pyrsgm_batch_set_tuning(void* matcher, const int tuning[4], char* err, int errlen);
+int pyrsgm_batch_get_tuning(void* matcher, int tuning[4]);
+int pyrsgm_batch_tune(void* matcher,
+                      const unsigned char* d_left, size_t left_pitch, size_t left_stride,
+                      const unsigned char* d_right, size_t right_pitch, size_t right_stride,
+                      short* d_disp16, size_t disp16_pitch, size_t disp16_stride,
+                      float* d_disp32, size_t disp32_pitch, size_t disp32_stride,
+                      float* d_conf, size_t conf_pitch, size_t conf_stride,
+                      hipStream_t stream, int reps, int best[4], double* best_ms, char* err,
+                      int errlen);
 size_t pyrsgm_batch_device_bytes(void* matcher);
 /* BatchMatcher::requiredBytes; 0 on invalid arguments (message in err). */
 size_t pyrsgm_batch_required_bytes(int width, int height, int batch,

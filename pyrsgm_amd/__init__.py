@@ -178,6 +178,10 @@ def lib() -> C.CDLL:
         L.pyrsgm_batch_required_bytes.argtypes = [ci, ci, ci, C.POINTER(_ParamsC), C.c_char_p, ci]
         L.pyrsgm_batch_required_bytes.restype = sz
         L.pyrsgm_batch_launch.argtypes = [vp, vp, C.c_char_p, ci]
+        L.pyrsgm_batch_set_tuning.argtypes = [vp, C.POINTER(ci), C.c_char_p, ci]
+        L.pyrsgm_batch_get_tuning.argtypes = [vp, C.POINTER(ci)]
+        L.pyrsgm_batch_tune.argtypes = ([vp] + [vp, sz, sz] * 5 +
+                                        [vp, ci, C.POINTER(ci), C.POINTER(C.c_double), C.c_char_p, ci])
         L.pyrsgm_batch_device_bytes.argtypes = [vp]
         L.pyrsgm_batch_device_bytes.restype = sz
         ll, ull = C.c_longlong, C.c_ulonglong
@@ -438,6 +442,33 @@ class BatchMatcher:
         err = C.create_string_buffer(512)
         if lib().pyrsgm_batch_launch(self._h, s.cuda_stream, err, 512) != 0:
             raise RuntimeError(err.value.decode())
+
+    _TUNING = ("census_rows", "paired", "xcd_remap", "side_stream")
+
+    def set_tuning(self, census_rows=4, paired=-1, xcd_remap=-1, side_stream=1):
+        """Launch shapes (every choice gives the same bits); drops a captured graph."""
+        t = (C.c_int * 4)(census_rows, paired, xcd_remap, side_stream)
+        err = C.create_string_buffer(512)
+        if lib().pyrsgm_batch_set_tuning(self._h, t, err, 512) != 0:
+            raise ValueError(err.value.decode())
+
+    def tuning(self) -> dict:
+        t = (C.c_int * 4)()
+        lib().pyrsgm_batch_get_tuning(self._h, t)
+        return dict(zip(self._TUNING, list(t)))
+
+    def tune(self, left, right, disp16=None, disp32=None, conf=None, reps=3, stream=None):
+        """Time candidate launch shapes on all pairs; keeps the fastest.
+        Returns (tuning, ms per batch)."""
+        import torch
+        s = stream if stream is not None else torch.cuda.current_stream()
+        args = self._args(left, right, disp16, disp32, conf, None)[:-1]
+        best, ms = (C.c_int * 4)(), C.c_double()
+        err = C.create_string_buffer(512)
+        if lib().pyrsgm_batch_tune(self._h, *args, s.cuda_stream, reps, best, C.byref(ms),
+                                   err, 512) != 0:
+            raise RuntimeError(err.value.decode())
+        return dict(zip(self._TUNING, list(best))), ms.value
 
     def device_memory_bytes(self) -> int:
         return lib().pyrsgm_batch_device_bytes(self._h)

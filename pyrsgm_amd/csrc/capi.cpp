@@ -219,7 +219,8 @@ hipError_t launchLevel(const LevelCall& c, Workspace& ws, hipStream_t s)
     a.exact = exactPathsNeeded(c);
     a.narrow = c.narrow;
     static const int remap = getenv("PYRSGM_NO_XCD_REMAP") ? 0 : 1;
-    a.xcdRemap = remap;
+    a.xcdRemap = c.tiles.xcdRemap == 0 ? remap : (c.tiles.xcdRemap == 2 ? 1 : 0);
+    a.pairedMode = c.tiles.paired;
     a.disp = ws.disp;
     a.conf = ws.conf;
     a.keys = ws.keys;
@@ -286,7 +287,7 @@ int computeSingleScale(
     int blockSize, int P1, int P2, int uniquenessRatio, int disp12MaxDiff, int preFilterCap,
     int costMode, int postFlags, int numPaths, int adaptiveP2, float p2Alpha, float p2Beta,
     float p2LowerMult, float p2UpperMult, const Batch& b, hipStream_t stream, int stages,
-    PathSink* sink)
+    PathSink* sink, const TileConfig* tiles)
 {
     if (!workspace || !d_left_gray || !d_right_gray || !geometryOk(width, height, numDisparities) ||
         !paramsOk(blockSize, P1, P2))
@@ -328,6 +329,7 @@ int computeSingleScale(
     c.narrow = narrowVolumes(c);
     c.stages = stages;
     c.sink = sink;
+    if (tiles) c.tiles = *tiles;
     c.batch = b;
     c.batch.ws = ws->slice;
     const hipError_t e = launchLevel(c, *ws, stream);
@@ -345,7 +347,8 @@ int computeLevel(
     int maxDisparity, int blockSize, int P1, int P2, int uniquenessRatio, int disp12MaxDiff,
     int preFilterCap, int costMode, int postFlags, int numPaths, int refineLevel,
     float minConfidence, int adaptiveP2, float p2Alpha, float p2Beta, float p2LowerMult,
-    float p2UpperMult, const Batch& b, hipStream_t stream, int stages, PathSink* sink)
+    float p2UpperMult, const Batch& b, hipStream_t stream, int stages, PathSink* sink,
+    const TileConfig* tiles)
 {
     if (!workspace || !d_left_gray || !d_right_gray || !geometryOk(width, height, numDisparities) ||
         (refineLevel && numDisparities > 256) || !paramsOk(blockSize, P1, P2))
@@ -400,6 +403,7 @@ int computeLevel(
     c.narrow = narrowVolumes(c);
     c.stages = stages;
     c.sink = sink;
+    if (tiles) c.tiles = *tiles;
     c.batch = b;
     c.batch.ws = ws->slice;
     const hipError_t e = launchLevel(c, *ws, stream);

@@ -66,6 +66,14 @@ size_t workspaceBytes(int W, int H, int D);
 int ensureWorkspace(Workspace* ws, int W, int H, int D, int n = 1);
 void releaseWorkspace(Workspace* ws);
 
+// Launch shapes a matcher may choose at run time (every choice gives the same
+// bits).  0 everywhere is the default shape.
+struct TileConfig {
+    int localRows = 0;  // rows per block of the refinement census cost: 0 (= 4), 2, 4 or 8
+    int paired = 0;     // paired lines for 16-label byte volumes: 0 by launch size, 1 off, 2 on
+    int xcdRemap = 0;   // XCD-contiguous T->B column groups: 0 default, 1 off, 2 on
+};
+
 // Everything one level needs; filled by the C-ABI entry points.
 struct LevelCall {
     const uint8_t* left; const uint8_t* right; int lstep, rstep;
@@ -81,6 +89,7 @@ struct LevelCall {
     int narrow;      // byte volumes (narrowVolumes)
     int stages;      // kStage* bits
     PathSink* sink;  // non-null: path launches are recorded here instead of issued
+    TileConfig tiles;
     Batch batch;
     int32_t lut[256];
 };
@@ -123,6 +132,7 @@ struct PathArgs {
     int exact;                 // pyramid: Cmax + 2*max(P2') may reach SHRT_MAX
     int narrow;                // byte volumes (see narrowVolumes)
     int xcdRemap;              // XCD-contiguous line groups (on unless PYRSGM_NO_XCD_REMAP is set)
+    int pairedMode;            // TileConfig::paired
     int nLRBlocks;             // combined launch: blocks [0, nLR) run L->R,
     int nRLBlocks;             // [nLR, nLR+nRL) R->L, the rest T->B
     int16_t* disp;
@@ -216,7 +226,8 @@ int computeSingleScale(void* workspace, const unsigned char* d_left_gray,
                        int disp12MaxDiff, int preFilterCap, int costMode, int postFlags,
                        int numPaths, int adaptiveP2, float p2Alpha, float p2Beta,
                        float p2LowerMult, float p2UpperMult, const Batch& b, hipStream_t stream,
-                       int stages = kStagesAll, PathSink* sink = nullptr);
+                       int stages = kStagesAll, PathSink* sink = nullptr,
+                       const TileConfig* tiles = nullptr);
 int computeLevel(void* workspace, const unsigned char* d_left_gray,
                  const unsigned char* d_right_gray, const short* d_parent_disp,
                  const float* d_parent_conf, short* d_disp, float* d_disp_float,
@@ -229,7 +240,8 @@ int computeLevel(void* workspace, const unsigned char* d_left_gray,
                  int preFilterCap, int costMode, int postFlags, int numPaths, int refineLevel,
                  float minConfidence, int adaptiveP2, float p2Alpha, float p2Beta,
                  float p2LowerMult, float p2UpperMult, const Batch& b, hipStream_t stream,
-                 int stages = kStagesAll, PathSink* sink = nullptr);
+                 int stages = kStagesAll, PathSink* sink = nullptr,
+                 const TileConfig* tiles = nullptr);
 // pyrDown of n pairs of (one or two) images; src / dst pair strides in bytes.
 hipError_t launchPyrDownBatch(const uint8_t* srcA, int sstepA, const uint8_t* srcB, int sstepB,
                               size_t sStride, int sw, int sh, uint8_t* dstA, int dstepA,

@@ -11,6 +11,7 @@
 // every thread produces 8 consecutive labels of one pixel and writes them with
 // one 16-byte store.
 #include <climits>
+#include <type_traits>
 
 #include "device_common.h"
 #include "internal.h"
@@ -304,21 +305,19 @@ __device__ __forceinline__ int32_t labelWord(int base, int valid, uint8_t im)
 }
 
 // ---- Census cost at refinement levels (sgm_kernels.cu:1112-1133) ------------
-// Block: NPX pixels x (Dp/8) label octets of kLocalRows rows.  A label's right
+// Block: NPX pixels x (Dp/8) label octets of ROWS rows (2, 4 or 8; 4 unless a
+// tuner picked another).  A label's right
 // column x - real lies in [x0 - maxD, x0 + NPX - 1 - minD]; when that span is
 // modest (STAGED) the right census words of the block's rows are staged in LDS
 // first, otherwise read from global memory.  Every global load of a thread
 // (staged words, parent disparity and confidence, left census word,
 // intensity, for all its rows) is issued before the first wait.
-constexpr int kLocalRows = 4;
-
-template <bool STAGED, bool NARROW>
+template <bool STAGED, bool NARROW, int ROWS>
 __global__ __launch_bounds__(256) void kCensusLocal(const uint32_t* cen, RefineIn r, int32_t* labels,
                                                     const uint8_t* img, int istep, int16_t* C,
                                                     int W, int H, int D, int Dp, int minD, int maxD,
                                                     Batch b)
 {
-    constexpr int ROWS = kLocalRows;
     constexpr int NPRE = 2;  // staged words per thread and row loaded ahead
     extern __shared__ uint32_t s_r[];
     const size_t pr = blockIdx.z;
@@ -933,26 +932,31 @@ hipError_t launchCost(const LevelCall& c, Workspace& ws, hipStream_t s)
     if (c.mode == kRefine) {
         RefineIn r{c.pdisp, c.pconf, c.pdpitch, c.pcpitch, c.PW, c.PH, c.minConf};
         dim3 grid((c.W + npx - 1) / npx, c.H, c.batch.n);
-        if (c.cost == 1) grid.y = (c.H + kLocalRows - 1) / kLocalRows;
         const uint32_t* cen = reinterpret_cast<const uint32_t*>(ws.desc);
         const long span = (long)npx + c.maxD - c.minD;  // staged right columns per row
-        const size_t lds = (size_t)span * 4 * kLocalRows;
-        if (c.cost == 1 && span <= kMaxStaged && c.narrow)
-            kCensusLocal<true, true><<<grid, 256, lds, s>>>(cen, r, ws.labels, c.left, c.lstep,
-                                                            ws.C, c.W, c.H, c.D, c.Dp, c.minD,
-                                                            c.maxD, c.batch);
-        else if (c.cost == 1 && span <= kMaxStaged)
-            kCensusLocal<true, false><<<grid, 256, lds, s>>>(cen, r, ws.labels, c.left, c.lstep,
-                                                             ws.C, c.W, c.H, c.D, c.Dp, c.minD,
-                                                             c.maxD, c.batch);
-        else if (c.cost == 1 && c.narrow)
-            kCensusLocal<false, true><<<grid, 256, 0, s>>>(cen, r, ws.labels, c.left, c.lstep,
-                                                           ws.C, c.W, c.H, c.D, c.Dp, c.minD,
-                                                           c.maxD, c.batch);
-        else if (c.cost == 1)
-            kCensusLocal<false, false><<<grid, 256, 0, s>>>(cen, r, ws.labels, c.left, c.lstep,
-                                                            ws.C, c.W, c.H, c.D, c.Dp, c.minD,
-                                                            c.maxD, c.batch);
+        if (c.cost == 1) {
+            const int rows = c.tiles.localRows ? c.tiles.localRows : 4;
+            auto census = [&](auto rowsC) {
+                constexpr int ROWS = decltype(rowsC)::value;
+                const dim3 g(grid.x, (c.H + ROWS - 1) / ROWS, grid.z);
+                const size_t lds = (size_t)span * 4 * ROWS;
+                if (span <= kMaxStaged && c.narrow)
+                    kCensusLocal<true, true, ROWS><<<g, 256, lds, s>>>(
+                        cen, r, ws.labels, c.left, c.lstep, ws.C, c.W, c.H, c.D, c.Dp, c.minD, c.maxD, c.batch);
+                else if (span <= kMaxStaged)
+                    kCensusLocal<true, false, ROWS><<<g, 256, lds, s>>>(
+                        cen, r, ws.labels, c.left, c.lstep, ws.C, c.W, c.H, c.D, c.Dp, c.minD, c.maxD, c.batch);
+                else if (c.narrow)
+                    kCensusLocal<false, true, ROWS><<<g, 256, 0, s>>>(
+                        cen, r, ws.labels, c.left, c.lstep, ws.C, c.W, c.H, c.D, c.Dp, c.minD, c.maxD, c.batch);
+                else
+                    kCensusLocal<false, false, ROWS><<<g, 256, 0, s>>>(
+                        cen, r, ws.labels, c.left, c.lstep, ws.C, c.W, c.H, c.D, c.Dp, c.minD, c.maxD, c.batch);
+            };
+            if (rows == 2) census(std::integral_constant<int, 2>{});
+            else if (rows == 8) census(std::integral_constant<int, 8>{});
+            else census(std::integral_constant<int, 4>{});
+        }
         else if (((c.cap > 15 ? c.cap : 15) | 1) <= 95 && c.Dp == 16 && c.D == 16 && half <= 3) {
             const dim3 tg((c.W + kTilePx - 1) / kTilePx, (c.H + kTileRows - 1) / kTileRows, c.batch.n);
             switch (half) {

@@ -237,6 +237,68 @@ int pyrsgm_batch_launch(void* matcher, hipStream_t stream, char* err, int errlen
     return -1;
 }
 
+namespace {
+pyrsgm::BatchTuning batchTuning(const int t[4])
+{
+    pyrsgm::BatchTuning b;
+    b.census_rows = t[0];
+    b.paired = t[1];
+    b.xcd_remap = t[2];
+    b.side_stream = t[3];
+    return b;
+}
+
+void batchTuningOut(const pyrsgm::BatchTuning& b, int t[4])
+{
+    t[0] = b.census_rows;
+    t[1] = b.paired;
+    t[2] = b.xcd_remap;
+    t[3] = b.side_stream;
+}
+} // namespace
+
+int pyrsgm_batch_set_tuning(void* matcher, const int tuning[4], char* err, int errlen)
+{
+    if (!matcher || !tuning) return -1;
+    try {
+        static_cast<pyrsgm::BatchMatcher*>(matcher)->setTuning(batchTuning(tuning));
+        return 0;
+    } catch (const std::exception& e) {
+        copyErr(e.what(), err, errlen);
+    }
+    return -1;
+}
+
+int pyrsgm_batch_get_tuning(void* matcher, int tuning[4])
+{
+    if (!matcher || !tuning) return -1;
+    batchTuningOut(static_cast<pyrsgm::BatchMatcher*>(matcher)->tuning(), tuning);
+    return 0;
+}
+
+int pyrsgm_batch_tune(void* matcher,
+                      const unsigned char* d_left, size_t left_pitch, size_t left_stride,
+                      const unsigned char* d_right, size_t right_pitch, size_t right_stride,
+                      short* d_disp16, size_t disp16_pitch, size_t disp16_stride,
+                      float* d_disp32, size_t disp32_pitch, size_t disp32_stride,
+                      float* d_conf, size_t conf_pitch, size_t conf_stride,
+                      hipStream_t stream, int reps, int best[4], double* best_ms, char* err,
+                      int errlen)
+{
+    if (!matcher) return -1;
+    try {
+        const pyrsgm::BatchTuning b = static_cast<pyrsgm::BatchMatcher*>(matcher)->tune(
+            {d_left, left_pitch, left_stride}, {d_right, right_pitch, right_stride},
+            {d_disp16, disp16_pitch, disp16_stride}, {d_disp32, disp32_pitch, disp32_stride},
+            {d_conf, conf_pitch, conf_stride}, stream, reps, best_ms);
+        if (best) batchTuningOut(b, best);
+        return 0;
+    } catch (const std::exception& e) {
+        copyErr(e.what(), err, errlen);
+    }
+    return -1;
+}
+
 size_t pyrsgm_batch_device_bytes(void* matcher)
 {
     return matcher ? static_cast<pyrsgm::BatchMatcher*>(matcher)->deviceMemoryBytes() : 0;

@@ -99,6 +99,8 @@ struct MatcherCore {
     // overlap saves (measured), and everything stays on the caller's stream.
     hipStream_t side = nullptr;
     hipEvent_t evFork = nullptr, evJoin = nullptr;
+    bool useSide = true;      // run the descriptor passes on `side` when it exists
+    pyr::TileConfig tiles{};  // launch shapes (all give the same bits)
 
     // Reference: stereo_matcher.cpp:169-193.  Fills info[0..levels].
     static void geometry(int w, int h, const Params& params,
@@ -239,6 +241,7 @@ struct MatcherCore {
         }
         pyrDownChain(L, R, stream, count);
         // Fork: the refinement levels' descriptor passes on the side stream.
+        hipStream_t side = useSide ? this->side : nullptr;
         if (side) {
             check(hipEventRecord(evFork, stream), "hipEventRecord");
             check(hipStreamWaitEvent(side, evFork, 0), "hipStreamWaitEvent");
@@ -280,7 +283,8 @@ struct MatcherCore {
             params.min_disparity, params.block_size, params.p1, params.p2,
             params.uniqueness_ratio, params.disp12_max_diff, params.pre_filter_cap, costMode,
             postFlags(0), params.num_paths, params.adaptive_p2 ? 1 : 0, params.p2_alpha,
-            params.p2_beta, params.p2_lower_mult, params.p2_upper_mult, b, stream, stages, sink);
+            params.p2_beta, params.p2_lower_mult, params.p2_upper_mult, b, stream, stages, sink,
+            &tiles);
         if (ret != 0) throwHip("single-scale SGM", hipGetLastError());
     }
 
@@ -364,7 +368,7 @@ struct MatcherCore {
                 params.disp12_max_diff, params.pre_filter_cap, costMode, postFlags(level),
                 params.num_paths, refine ? 1 : 0, params.pyramid_min_confidence,
                 params.adaptive_p2 ? 1 : 0, params.p2_alpha, params.p2_beta, params.p2_lower_mult,
-                params.p2_upper_mult, b, stream, stages, sink);
+                params.p2_upper_mult, b, stream, stages, sink, &tiles);
             if (ret != 0) {
                 std::ostringstream oss;
                 oss << "pyramid level " << level << " (" << li.width << "x" << li.height

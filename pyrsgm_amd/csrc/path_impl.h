@@ -1112,7 +1112,8 @@ hipError_t dispatchPaths(const PathArgs& a, hipStream_t s)
         // Paired lines when the launch has at least four 4-line waves per SIMD
         // of the 1024: the recurrence wave is then bound by VALU issue, and two
         // lines per lane in 16-bit halves nearly halve its instructions per line.
-        if (a.Dp == 16 && (size_t)blocks * a.batch.n >= kPairedWaves) {
+        if (a.Dp == 16 && (a.pairedMode == 2 ||
+                           (a.pairedMode == 0 && (size_t)blocks * a.batch.n >= kPairedWaves))) {
             PathArgs c = a;
             c.nLRBlocks = a.numPaths >= 2 ? (a.H + 7) / 8 : 0;
             c.nRLBlocks = (a.H + 7) / 8;

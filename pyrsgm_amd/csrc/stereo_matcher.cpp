@@ -10,6 +10,8 @@
 
 #include <algorithm>
 #include <array>
+#include <map>
+#include <mutex>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -95,13 +97,117 @@ LevelInfo StereoMatcher::levelInfo(int level) const
 size_t StereoMatcher::deviceMemoryBytes() const { return impl_->deviceBytes(); }
 
 // ---- BatchMatcher ---------------------------------------------------------
+namespace {
+
+// Tunings found by BatchMatcher::tune, per (size, batch, Params), for the life
+// of the process.
+std::mutex g_batchTuneMutex;
+std::map<std::string, BatchTuning> g_batchTuned;
+
+std::string batchKey(int w, int h, int n, const Params& p)
+{
+    std::ostringstream o;
+    o << w << 'x' << h << 'x' << n << ':' << p.num_disparities << ':' << p.min_disparity << ':'
+      << p.block_size << ':' << (int)p.cost_mode << ':' << p.p1 << ':' << p.p2 << ':'
+      << p.uniqueness_ratio << ':' << p.disp12_max_diff << ':' << p.lr_check << ':'
+      << p.median_filter << ':' << p.num_paths << ':' << p.pre_filter_cap << ':'
+      << p.pyramid_levels << ':' << p.pyramid_coarse_disparities << ':'
+      << p.pyramid_max_disparity << ':' << p.pyramid_min_confidence << ':' << p.adaptive_p2
+      << ':' << p.p2_alpha << ':' << p.p2_beta << ':' << p.p2_lower_mult << ':'
+      << p.p2_upper_mult;
+    return o.str();
+}
+
+void checkBatchTuning(const BatchTuning& t)
+{
+    if (t.census_rows != 2 && t.census_rows != 4 && t.census_rows != 8)
+        throw std::invalid_argument("pyrsgm: BatchTuning.census_rows must be 2, 4 or 8");
+    if (t.paired < -1 || t.paired > 1 || t.xcd_remap < -1 || t.xcd_remap > 1)
+        throw std::invalid_argument("pyrsgm: BatchTuning.paired / xcd_remap must be -1, 0 or 1");
+    if (t.side_stream != 0 && t.side_stream != 1)
+        throw std::invalid_argument("pyrsgm: BatchTuning.side_stream must be 0 or 1");
+}
+
+} // namespace
+
 struct BatchMatcher::Impl : MatcherCore {
     hipStream_t captureStream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 
-    Impl(int w, int h, int n, const Params& p) : MatcherCore(w, h, n, p) {}
+    BatchTuning tun;
+
+    Impl(int w, int h, int n, const Params& p) : MatcherCore(w, h, n, p)
+    {
+        std::lock_guard<std::mutex> lock(g_batchTuneMutex);
+        auto it = g_batchTuned.find(batchKey(width, height, this->n, params));
+        if (it != g_batchTuned.end()) apply(it->second);
+    }
     ~Impl() { dropGraph(); if (captureStream) (void)hipStreamDestroy(captureStream); }
+
+    void apply(const BatchTuning& t)
+    {
+        checkBatchTuning(t);
+        tun = t;
+        tiles.localRows = t.census_rows;
+        tiles.paired = t.paired < 0 ? 0 : (t.paired ? 2 : 1);
+        tiles.xcdRemap = t.xcd_remap < 0 ? 0 : (t.xcd_remap ? 2 : 1);
+        useSide = t.side_stream != 0;
+    }
+
+    double timePasses(BatchBuffer<const uint8_t> L, BatchBuffer<const uint8_t> R,
+                      BatchBuffer<int16_t> d16, BatchBuffer<float> d32, BatchBuffer<float> cf,
+                      hipStream_t s, int reps)
+    {
+        hipEvent_t a = nullptr, b = nullptr;
+        check(hipEventCreate(&a), "hipEventCreate");
+        check(hipEventCreate(&b), "hipEventCreate");
+        compute(L, R, d16, d32, cf, s, -1);  // warm-up
+        check(hipEventRecord(a, s), "hipEventRecord");
+        for (int r = 0; r < reps; ++r) compute(L, R, d16, d32, cf, s, -1);
+        check(hipEventRecord(b, s), "hipEventRecord");
+        check(hipEventSynchronize(b), "hipEventSynchronize");
+        float ms = 0;
+        check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        return (double)ms / reps;
+    }
+
+    // Coordinate descent from the current tuning over each knob in turn.
+    BatchTuning tune(BatchBuffer<const uint8_t> L, BatchBuffer<const uint8_t> R,
+                     BatchBuffer<int16_t> d16, BatchBuffer<float> d32, BatchBuffer<float> cf,
+                     hipStream_t s, int reps, double* bestMs)
+    {
+        if (reps < 1) reps = 1;
+        dropGraph();
+        BatchTuning best = tun;
+        double bestT = timePasses(L, R, d16, d32, cf, s, reps);
+        auto consider = [&](const BatchTuning& t) {
+            apply(t);
+            const double ms = timePasses(L, R, d16, d32, cf, s, reps);
+            if (ms < bestT) {
+                bestT = ms;
+                best = t;
+            }
+        };
+        for (int rows : {2, 4, 8})
+            if (rows != best.census_rows) { BatchTuning t = best; t.census_rows = rows; consider(t); }
+        for (int pk : {-1, 0, 1})
+            if (pk != best.paired) { BatchTuning t = best; t.paired = pk; consider(t); }
+        for (int xr : {-1, 0, 1})
+            if (xr != best.xcd_remap) { BatchTuning t = best; t.xcd_remap = xr; consider(t); }
+        if (side)
+            for (int sd : {0, 1})
+                if (sd != best.side_stream) { BatchTuning t = best; t.side_stream = sd; consider(t); }
+        apply(best);
+        {
+            std::lock_guard<std::mutex> lock(g_batchTuneMutex);
+            g_batchTuned[batchKey(width, height, n, params)] = best;
+        }
+        if (bestMs) *bestMs = bestT;
+        return best;
+    }
 
     void dropGraph()
     {
@@ -161,6 +267,23 @@ void BatchMatcher::launch(hipStream_t stream)
 {
     if (!impl_->exec) throw std::logic_error("pyrsgm: BatchMatcher::launch before capture");
     check(hipGraphLaunch(impl_->exec, stream), "hipGraphLaunch");
+}
+
+void BatchMatcher::setTuning(const BatchTuning& t)
+{
+    impl_->dropGraph();
+    impl_->apply(t);
+}
+
+BatchTuning BatchMatcher::tuning() const { return impl_->tun; }
+
+BatchTuning BatchMatcher::tune(BatchBuffer<const uint8_t> left, BatchBuffer<const uint8_t> right,
+                               BatchBuffer<int16_t> disp16, BatchBuffer<float> disp32,
+                               BatchBuffer<float> conf, hipStream_t stream, int reps,
+                               double* best_ms)
+{
+    if (!left.data || !right.data) throw std::invalid_argument("pyrsgm: null input image");
+    return impl_->tune(left, right, disp16, disp32, conf, stream, reps, best_ms);
 }
 
 int BatchMatcher::batch() const { return impl_->n; }

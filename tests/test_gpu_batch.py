@@ -189,3 +189,52 @@ def test_batch_paired_lines_equal_single_pairs(n, w, h, D, levels):
         U.assert_same(f"pair {i} conf", cf[i].cpu().numpy(), want[2])
     ref = O.matcher_compute(lefts[1], rights[1], params)
     U.assert_same("pair 1 vs oracle", d16[1].cpu().numpy(), ref.disp16)
+
+
+@pytest.mark.parametrize("census_rows,paired,xcd_remap,side_stream",
+                         [(2, 1, 0, 0), (8, 0, 1, 1), (4, -1, -1, 1), (2, -1, 1, 0)])
+def test_batch_tunings_equal_single_pairs(census_rows, paired, xcd_remap, side_stream):
+    """Every launch shape (census rows per block, paired lines forced on or off,
+    XCD remap, side stream) gives the same bits as the pairs matched alone."""
+    n, w, h, D = 3, 241, 135, 128
+    lefts, rights = pairs(n, w, h, D - 1, seed0=900)
+    params = P.validate_params(P.Params(num_disparities=D, pyramid_levels=2,
+                                        cost_mode=P.CostMode.Census, p1=8, p2=32,
+                                        adaptive_p2=True, median_filter=True))
+    bm = P.BatchMatcher(w, h, n, params)
+    bm.set_tuning(census_rows, paired, xcd_remap, side_stream)
+    assert bm.tuning() == {"census_rows": census_rows, "paired": paired, "xcd_remap": xcd_remap,
+                           "side_stream": side_stream}
+    print("case: batch tuning", bm.tuning(), flush=True)
+    d16, d32, cf = run_batch(bm, batch_tensor(lefts, torch.uint8), batch_tensor(rights, torch.uint8),
+                             n, h, w)
+    want = singles(lefts, rights, params)
+    for i in range(n):
+        U.assert_same(f"pair {i} disp16", d16[i].cpu().numpy(), want[i][0])
+        U.assert_same(f"pair {i} conf", cf[i].cpu().numpy(), want[i][2])
+    bm.close()
+
+
+def test_batch_tune_keeps_results():
+    n, w, h, D = 4, 200, 120, 64
+    lefts, rights = pairs(n, w, h, D - 1, seed0=950)
+    params = P.validate_params(P.Params(num_disparities=D, pyramid_levels=1,
+                                        cost_mode=P.CostMode.Census, p1=8, p2=32,
+                                        adaptive_p2=True))
+    bm = P.BatchMatcher(w, h, n, params)
+    L, R = batch_tensor(lefts, torch.uint8), batch_tensor(rights, torch.uint8)
+    d16 = out_tensor(n, h, w, torch.int16, 0)
+    print("case: batch tune", flush=True)
+    best, ms = bm.tune(L, R, d16, reps=2)
+    assert ms > 0 and bm.tuning() == best and best["census_rows"] in (2, 4, 8)
+    with pytest.raises(ValueError):
+        bm.set_tuning(census_rows=3)
+    d16, d32, cf = run_batch(bm, L, R, n, h, w)
+    want = singles(lefts, rights, params)
+    for i in range(n):
+        U.assert_same(f"tuned pair {i} disp16", d16[i].cpu().numpy(), want[i][0])
+    # A new matcher of the same key starts with the tuned shapes.
+    bm2 = P.BatchMatcher(w, h, n, params)
+    assert bm2.tuning() == best
+    bm2.close()
+    bm.close()

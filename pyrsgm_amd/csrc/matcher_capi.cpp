@@ -76,6 +76,26 @@ pyrsgm_params_c fromParams(const pyrsgm::Params& p)
 
 } // namespace
 
+namespace {
+pyrsgm::BatchTuning batchTuning(const int t[4])
+{
+    pyrsgm::BatchTuning b;
+    b.census_rows = t[0];
+    b.paired = t[1];
+    b.xcd_remap = t[2];
+    b.side_stream = t[3];
+    return b;
+}
+
+void batchTuningOut(const pyrsgm::BatchTuning& b, int t[4])
+{
+    t[0] = b.census_rows;
+    t[1] = b.paired;
+    t[2] = b.xcd_remap;
+    t[3] = b.side_stream;
+}
+} // namespace
+
 extern "C" {
 
 int pyrsgm_validate_params(const pyrsgm_params_c* in, pyrsgm_params_c* out, char* err, int errlen)
@@ -237,25 +257,6 @@ int pyrsgm_batch_launch(void* matcher, hipStream_t stream, char* err, int errlen
     return -1;
 }
 
-namespace {
-pyrsgm::BatchTuning batchTuning(const int t[4])
-{
-    pyrsgm::BatchTuning b;
-    b.census_rows = t[0];
-    b.paired = t[1];
-    b.xcd_remap = t[2];
-    b.side_stream = t[3];
-    return b;
-}
-
-void batchTuningOut(const pyrsgm::BatchTuning& b, int t[4])
-{
-    t[0] = b.census_rows;
-    t[1] = b.paired;
-    t[2] = b.xcd_remap;
-    t[3] = b.side_stream;
-}
-} // namespace
 
 int pyrsgm_batch_set_tuning(void* matcher, const int tuning[4], char* err, int errlen)
 {

@@ -33,6 +33,8 @@
 //    scale paths the initial L = C (sgm_kernels.cu:547, :607, :702).
 #pragma once
 
+#include <type_traits>
+
 #include "device_common.h"
 #include "internal.h"
 
@@ -863,7 +865,8 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
         *reinterpret_cast<uint8_t*>(tpB) = (uint8_t)(Hc >> 16);
     };
 
-    auto processPK = [&](const CBuf& c, int k, bool full) {
+    auto processPK = [&](const CBuf& c, int k, auto fullC) {
+        constexpr bool full = decltype(fullC)::value;
         const int sc = chunkStart(k);
         int shvA = 0, shvB = 0;
         uint32_t moveMask = 0, bigMask = 0, p2v = 0;
@@ -920,8 +923,10 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
         unrollSteps<U>(run);
     };
 
-    // Process chunk k; full: all U steps exist.
-    auto process = [&](const CBuf& c, int k, bool full) {
+    // Process chunk k; full (a compile-time constant, so that full chunks carry
+    // no per-step guard): all U steps exist.
+    auto process = [&](const CBuf& c, int k, auto fullC) {
+        constexpr bool full = decltype(fullC)::value;
         const int sc = chunkStart(k);
         int p2v = 0, shv = 0;
         uint32_t moveMask = 0, bigMask = 0;
@@ -975,11 +980,11 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
         unrollSteps<U>(run);
     };
 
-    auto proc = [&](const CBuf& c, int k, bool full) {
+    auto proc = [&](const CBuf& c, int k, auto fullC) {
         if constexpr (PK)
-            processPK(c, k, full);
+            processPK(c, k, fullC);
         else
-            process(c, k, full);
+            process(c, k, fullC);
     };
     // Chunk k+1 is read from LDS while chunk k is processed; barrier k+1 ends
     // chunk k (its tile complete, slot k free for the memory wave).
@@ -989,15 +994,18 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
     int k = 0;
     for (; k + 2 <= nFull; k += 2) {
         cread(cB, k + 1);
-        proc(cA, k, true);
+        proc(cA, k, std::true_type{});
         wgBarrier();
         if (k + 2 < nChunks) cread(cA, k + 2);
-        proc(cB, k + 1, true);
+        proc(cB, k + 1, std::true_type{});
         wgBarrier();
     }
     for (; k < nChunks; ++k) {
         if (k + 1 < nChunks) cread(cB, k + 1);
-        proc(cA, k, chunkFull(k));
+        if (chunkFull(k))
+            proc(cA, k, std::true_type{});
+        else
+            proc(cA, k, std::false_type{});
         wgBarrier();
         cA = cB;
     }

@@ -70,6 +70,13 @@ def test_library_exports_the_cpp_api():
                 "pyrsgm::BatchMatcher::launch(",
                 "pyrsgm::BatchMatcher::batch() const",
                 "pyrsgm::BatchMatcher::requiredBytes(int, int, int, pyrsgm::Params const&)",
+                "pyrsgm::BatchMatcher::tune(",
+                "pyrsgm::BatchMatcher::setTuning(pyrsgm::BatchTuning const&)",
+                "pyrsgm::MixedBatch::MixedBatch(",
+                "pyrsgm::MixedBatch::compute(pyrsgm_pair_io_c const*, ihipStream_t*)",
+                "pyrsgm::MixedBatch::capture(pyrsgm_pair_io_c const*)",
+                "pyrsgm::MixedBatch::launch(ihipStream_t*)",
+                "pyrsgm::MixedBatch::tune(",
                 "pyrsgm::Server::Server(pyrsgm::ServeOptions const&)",
                 "pyrsgm::Server::submit(pyrsgm::ServeRequest const&, long)",
                 "pyrsgm::Server::wait(unsigned long)",
@@ -88,6 +95,20 @@ def test_batch_rejects_bad_sizes_on_cpu():
         P.BatchMatcher(0, 32, 2, P.Params(num_disparities=64))
     with pytest.raises(ValueError):
         P.BatchMatcher(64, 32, 2, P.Params(p1=10, p2=5))
+
+
+def test_mixed_rejects_bad_pairs_on_cpu():
+    # Pair list and parameter checks run before any device call.
+    with pytest.raises(ValueError):
+        P.MixedBatch([])
+    with pytest.raises(ValueError):
+        P.MixedBatch([(0, 32, P.Params(num_disparities=64))])
+    with pytest.raises(ValueError):
+        P.MixedBatch([(64, 32, P.Params(num_disparities=64)), (64, 32, P.Params(p1=10, p2=5))])
+    L = P.lib()
+    assert L.pyrsgm_mixed_compute(None, None, None, None, 0) == -1
+    assert L.pyrsgm_mixed_launch(None, None, None, 0) == -1
+    assert L.pyrsgm_mixed_device_bytes(None) == 0
 
 
 def test_batch_required_bytes_on_cpu():

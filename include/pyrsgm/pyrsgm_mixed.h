@@ -9,10 +9,11 @@
  * (step s is level levels_i - s of pair i):
  *   - the path recurrences of all pairs at a step that share a kernel
  *     specialisation run as ONE persistent launch whose workgroups take
- *     (pair level, direction, line group) items from a device queue;
+ *     (pair level, direction, line group) items from a device queue, and
+ *     their winner-take-all as ONE launch over the same task table;
  *   - the per-pair stages around them (pyrDown, descriptors, costs,
- *     winner-take-all, post-processing) run on parallel streams that join
- *     before and fork after each queued launch.
+ *     post-processing) run on parallel streams that join before and fork
+ *     after each step's queued launches.
  * capture() records the whole pass into one HIP graph.  tune() times
  * candidate launch shapes (streams for the per-pair stages, resident
  * workgroups per CU of the queued launches, paired lines) on the device and
@@ -71,7 +72,8 @@ int pyrsgm_mixed_get_tuning(void* batch, pyrsgm_mixed_tuning_c* out);
  * captured graph. */
 int pyrsgm_mixed_tune(void* batch, const pyrsgm_pair_io_c* io, hipStream_t stream, int reps,
                       pyrsgm_mixed_tuning_c* best, double* best_ms, char* err, int errlen);
-/* out[4] = {queued path launches, tasks in them, work items in them, per-pair launches} */
+/* out[4] = {queued path launches (each followed by one queued winner-take-all
+ * launch), tasks in them, work items in them, per-pair stage launches} */
 int pyrsgm_mixed_plan(void* batch, long long out[4]);
 size_t pyrsgm_mixed_device_bytes(void* batch);
 

@@ -195,6 +195,12 @@ hipError_t launchLevel(const LevelCall& c, Workspace& ws, hipStream_t s)
         if (e != hipSuccess) return e;
     }
     if (!(c.stages & (kStagePaths | kStageFinish))) return hipSuccess;
+    if (!(c.stages & (kStagePaths | kStageWTA))) {
+        profBegin(PYRSGM_STAGE_POST, c.W, c.H, c.D, s);
+        e = launchPost(c, ws, s);
+        profEnd(s);
+        return e;
+    }
 
     PathArgs a;
     memset(&a, 0, sizeof(a));
@@ -239,11 +245,13 @@ hipError_t launchLevel(const LevelCall& c, Workspace& ws, hipStream_t s)
             if (e != hipSuccess) return e;
         }
     }
-    if (!(c.stages & kStageFinish)) return hipSuccess;
-    profBegin(PYRSGM_STAGE_WTA, c.W, c.H, c.D, s);
-    e = launchWTA(a, c.mode, s);
-    profEnd(s);
-    if (e != hipSuccess) return e;
+    if (c.stages & kStageWTA) {
+        profBegin(PYRSGM_STAGE_WTA, c.W, c.H, c.D, s);
+        e = launchWTA(a, c.mode, s);
+        profEnd(s);
+        if (e != hipSuccess) return e;
+    }
+    if (!(c.stages & kStagePost)) return hipSuccess;
     profBegin(PYRSGM_STAGE_POST, c.W, c.H, c.D, s);
     e = launchPost(c, ws, s);
     profEnd(s);

@@ -154,7 +154,8 @@ hipError_t launchWTA(const PathArgs& a, int mode, hipStream_t s);
 // intervals and the key reset, which read only the level's images), the cost
 // volume, the path recurrences, and the finish (winner-take-all and
 // post-processing).
-constexpr int kStagePrep = 1, kStageCost = 2, kStagePaths = 4, kStageFinish = 8;
+constexpr int kStagePrep = 1, kStageCost = 2, kStagePaths = 4, kStageWTA = 8, kStagePost = 16;
+constexpr int kStageFinish = kStageWTA | kStagePost;
 constexpr int kStageRest = kStageCost | kStagePaths | kStageFinish;
 constexpr int kStagesAll = kStagePrep | kStageRest;
 
@@ -203,6 +204,16 @@ hipError_t launchPathsQueuedCoarse(const QueueArgs& q, int Dp, int adaptive, int
                                    int narrow, int paired, int grid, hipStream_t s);
 hipError_t launchPathsQueuedRefine(const QueueArgs& q, int Dp, int adaptive, int exact,
                                    int narrow, int paired, int grid, hipStream_t s);
+// Winner-take-all of many pairs' levels of one specialisation (mode, Dp,
+// byte volumes) in one launch: block b belongs to task t with
+// start[t] <= b < start[t + 1], task t having wtaBlocks(tasks[t]) blocks.
+struct WtaQueueArgs {
+    const PathArgs* tasks;
+    const int* start;  // device [nTasks + 1]
+    int nTasks, total;
+};
+int wtaBlocks(const PathArgs& a);
+hipError_t launchWTAQueued(const WtaQueueArgs& q, int mode, int Dp, int narrow, hipStream_t s);
 // Sets tasks[idx[i]].img / .istep (the per-call left image of single-scale
 // tasks with adaptive P2) on the device.
 constexpr int kMaxImgPatches = 64;

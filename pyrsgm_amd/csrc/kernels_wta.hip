@@ -39,13 +39,13 @@ __device__ __forceinline__ void load16(int (&v)[16], const int16_t* p)
     }
 }
 
+// One block of pixels (block bx of the level, pair pr).
 template <int Q, int MODE, bool NARROW>
-__global__ __launch_bounds__(256) void kWTA(PathArgs a)
+__device__ __forceinline__ void wtaBlock(const PathArgs& a, size_t pr, size_t bx)
 {
     constexpr bool PYR = MODE != kSingle;
     const int W = a.W, D = a.D;
     const size_t P = (size_t)W * a.H;
-    const size_t pr = blockIdx.y;  // pair
     const size_t wsS = a.batch.ws;
     const int16_t* const Arl = pairAt(a.Arl, pr, wsS);
     const int16_t* const Alr = pairAt(a.Alr, pr, wsS);
@@ -54,7 +54,7 @@ __global__ __launch_bounds__(256) void kWTA(PathArgs a)
     int16_t* const dispV = pairAt(a.disp, pr, wsS);
     float* const confV = pairAt(a.conf, pr, wsS);
     uint32_t* const keys = pairAt(a.keys, pr, wsS);
-    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t t = bx * 256 + threadIdx.x;
     const size_t p = t / Q;
     const int sub = (int)(t - p * Q);
     const bool ok = p < P;
@@ -262,6 +262,53 @@ __global__ __launch_bounds__(256) void kWTA(PathArgs a)
     confV[p] = cf;
 }
 
+template <int Q, int MODE, bool NARROW>
+__global__ __launch_bounds__(256) void kWTA(PathArgs a)
+{
+    wtaBlock<Q, MODE, NARROW>(a, blockIdx.y, blockIdx.x);
+}
+
+// Many pairs' levels of one specialisation in one launch: block b belongs to
+// task t with start[t] <= b < start[t + 1] (the tasks' own block counts).
+template <int Q, int MODE, bool NARROW>
+__global__ __launch_bounds__(256) void kWTAQ(WtaQueueArgs q)
+{
+    const int b = blockIdx.x;
+    int lo = 0, hi = q.nTasks - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (q.start[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    wtaBlock<Q, MODE, NARROW>(q.tasks[lo], 0, (size_t)(b - q.start[lo]));
+}
+
+template <int MODE, bool NARROW>
+hipError_t dispatchWTAQ(const WtaQueueArgs& q, int Dp, hipStream_t s)
+{
+    const dim3 grid((unsigned)q.total);
+    if constexpr (NARROW) {
+        switch (Dp / 16) {
+        case 1: kWTAQ<1, MODE, true><<<grid, 256, 0, s>>>(q); break;
+        case 2: kWTAQ<2, MODE, true><<<grid, 256, 0, s>>>(q); break;
+        case 4: kWTAQ<4, MODE, true><<<grid, 256, 0, s>>>(q); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
+    switch (Dp / 16) {
+    case 1: kWTAQ<1, MODE, false><<<grid, 256, 0, s>>>(q); break;
+    case 2: kWTAQ<2, MODE, false><<<grid, 256, 0, s>>>(q); break;
+    case 4: kWTAQ<4, MODE, false><<<grid, 256, 0, s>>>(q); break;
+    case 8: kWTAQ<8, MODE, false><<<grid, 256, 0, s>>>(q); break;
+    case 16: kWTAQ<16, MODE, false><<<grid, 256, 0, s>>>(q); break;
+    case 32: kWTAQ<32, MODE, false><<<grid, 256, 0, s>>>(q); break;
+    case 64: kWTAQ<64, MODE, false><<<grid, 256, 0, s>>>(q); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 template <int MODE, bool NARROW>
 hipError_t dispatchWTA(const PathArgs& a, hipStream_t s)
 {
@@ -298,6 +345,19 @@ hipError_t dispatchWTAMode(const PathArgs& a, hipStream_t s)
 }
 
 } // namespace
+
+int wtaBlocks(const PathArgs& a)
+{
+    return (int)(((size_t)a.W * a.H * (size_t)(a.Dp / 16) + 255) / 256);
+}
+
+hipError_t launchWTAQueued(const WtaQueueArgs& q, int mode, int Dp, int narrow, hipStream_t s)
+{
+    if (q.total <= 0) return hipSuccess;
+    if (mode == kRefine) return narrow ? dispatchWTAQ<kRefine, true>(q, Dp, s) : dispatchWTAQ<kRefine, false>(q, Dp, s);
+    if (mode == kCoarse) return narrow ? dispatchWTAQ<kCoarse, true>(q, Dp, s) : dispatchWTAQ<kCoarse, false>(q, Dp, s);
+    return narrow ? dispatchWTAQ<kSingle, true>(q, Dp, s) : dispatchWTAQ<kSingle, false>(q, Dp, s);
+}
 
 hipError_t launchWTA(const PathArgs& a, int mode, hipStream_t s)
 {

@@ -1,6 +1,7 @@
 // mixed.cpp -- pyrsgm::MixedBatch: pairs of different sizes and parameters in
-// one fused pass (queued path launches over all pairs, per-pair stages on
-// parallel streams, whole pass capturable as one HIP graph), and its tuner.
+// one fused pass (queued path and winner-take-all launches over all pairs,
+// per-pair stages on parallel streams, whole pass capturable as one HIP
+// graph), and its tuner.
 #include <string.h>
 
 #include <algorithm>
@@ -44,6 +45,7 @@ struct MixedBatch::Impl {
         int step = 0;
         int mode = 0, Dp = 0, adaptive = 0, exact = 0, narrow = 0, paired = 0;
         int taskOff = 0, nTasks = 0, total = 0;
+        int wtaTotal = 0;  // blocks of the group's winner-take-all launch
         int grid = 0;
         std::vector<int> patchTask, patchPair;  // single-scale adaptive tasks (per-call image)
     };
@@ -61,10 +63,13 @@ struct MixedBatch::Impl {
     std::vector<pyr::PathArgs> hostTasks;  // device table image
     std::vector<int> hostStart;            // per group nTasks + 1 entries, concatenated
     std::vector<size_t> startOff;          // group -> offset into hostStart
+    std::vector<int> hostWtaStart;         // per group nTasks + 1 WTA block offsets, concatenated
+    std::vector<size_t> wtaStartOff;       // group -> offset into hostWtaStart
     void* dev = nullptr;
     size_t devBytes = 0;
     pyr::PathArgs* dTasks = nullptr;
     int* dStart = nullptr;
+    int* dWtaStart = nullptr;
     uint32_t* dCounters = nullptr;
     int numCUs = 256;
     std::vector<int> branchOf;
@@ -165,6 +170,8 @@ struct MixedBatch::Impl {
         hostTasks.clear();
         hostStart.clear();
         startOff.clear();
+        hostWtaStart.clear();
+        wtaStartOff.clear();
         perPairLaunches = 0;
         for (int step = 0; step < maxSteps; ++step) {
             // Tasks of this step, grouped by kernel specialisation.
@@ -209,7 +216,8 @@ struct MixedBatch::Impl {
                 const int lpb = pyr::queuedLinesPerBlock(g.Dp, g.paired);
                 g.taskOff = (int)hostTasks.size();
                 startOff.push_back(hostStart.size());
-                int total = 0;
+                wtaStartOff.push_back(hostWtaStart.size());
+                int total = 0, wtaTotal = 0;
                 for (int r : rs) {
                     pyr::PathArgs a = recorded[r].t.a;
                     a.nLRBlocks = a.numPaths >= 2 ? (a.H + lpb - 1) / lpb : 0;
@@ -221,25 +229,30 @@ struct MixedBatch::Impl {
                     }
                     hostStart.push_back(total);
                     total += a.nLRBlocks + a.nRLBlocks + nTB;
+                    hostWtaStart.push_back(wtaTotal);
+                    wtaTotal += pyr::wtaBlocks(a);
                     hostTasks.push_back(a);
                 }
                 hostStart.push_back(total);
+                hostWtaStart.push_back(wtaTotal);
                 g.nTasks = (int)rs.size();
                 g.total = total;
+                g.wtaTotal = wtaTotal;
                 const int occ = pyr::queuedBlocksPerCU(g.mode, g.Dp, g.adaptive, g.exact, g.narrow, g.paired);
                 const int bpc = tun.blocks_per_cu > 0 ? tun.blocks_per_cu : occ;
                 g.grid = (int)std::min<long long>((long long)total, (long long)numCUs * bpc);
                 groups.push_back(std::move(g));
             }
             for (int i = 0; i < (int)cores.size(); ++i)
-                if (levelAt(i, step) >= 0) perPairLaunches += 4;  // prep, cost, WTA, post
+                if (levelAt(i, step) >= 0) perPairLaunches += 3;  // prep, cost, post
         }
         for (auto& c : cores) perPairLaunches += c->levels;  // pyrDown
         // Device tables: tasks, starts, two counters per group.
         const size_t tasksBytes = (hostTasks.size() * sizeof(pyr::PathArgs) + 255) / 256 * 256;
         const size_t startBytes = (hostStart.size() * sizeof(int) + 255) / 256 * 256;
+        const size_t wtaBytes = (hostWtaStart.size() * sizeof(int) + 255) / 256 * 256;
         const size_t counterBytes = groups.size() * 2 * sizeof(uint32_t);
-        const size_t need = tasksBytes + startBytes + counterBytes;
+        const size_t need = tasksBytes + startBytes + wtaBytes + counterBytes;
         if (need > devBytes) {
             if (dev) check(hipFree(dev), "hipFree");
             dev = nullptr;
@@ -250,11 +263,14 @@ struct MixedBatch::Impl {
         char* base = static_cast<char*>(dev);
         dTasks = reinterpret_cast<pyr::PathArgs*>(base);
         dStart = reinterpret_cast<int*>(base + tasksBytes);
-        dCounters = reinterpret_cast<uint32_t*>(base + tasksBytes + startBytes);
+        dWtaStart = reinterpret_cast<int*>(base + tasksBytes + startBytes);
+        dCounters = reinterpret_cast<uint32_t*>(base + tasksBytes + startBytes + wtaBytes);
         check(hipMemcpy(dTasks, hostTasks.data(), hostTasks.size() * sizeof(pyr::PathArgs),
                         hipMemcpyHostToDevice), "hipMemcpy");
         check(hipMemcpy(dStart, hostStart.data(), hostStart.size() * sizeof(int), hipMemcpyHostToDevice),
               "hipMemcpy");
+        check(hipMemcpy(dWtaStart, hostWtaStart.data(), hostWtaStart.size() * sizeof(int),
+                        hipMemcpyHostToDevice), "hipMemcpy");
         check(hipMemset(dCounters, 0, counterBytes), "hipMemset");
         check(hipDeviceSynchronize(), "hipDeviceSynchronize");
         balanceBranches();
@@ -371,10 +387,19 @@ struct MixedBatch::Impl {
                     oss << "queued paths (step " << step << ", labels " << g.Dp << ", " << g.nTasks << " tasks)";
                     throwHip(oss.str(), e);
                 }
+                pyr::WtaQueueArgs wq;
+                wq.tasks = dTasks + g.taskOff;
+                wq.start = dWtaStart + wtaStartOff[gi];
+                wq.nTasks = g.nTasks;
+                wq.total = g.wtaTotal;
+                pyr::profBegin(PYRSGM_STAGE_WTA, 0, 0, g.Dp, main);
+                const hipError_t ew = pyr::launchWTAQueued(wq, g.mode, g.Dp, g.narrow, main);
+                pyr::profEnd(main);
+                if (ew != hipSuccess) throwHip("queued winner-take-all", ew);
             }
             fork(main);
             for (int i = 0; i < n; ++i)
-                if (levelAt(i, step) >= 0) pairStages(i, levelAt(i, step), io[i], br(i), pyr::kStageFinish);
+                if (levelAt(i, step) >= 0) pairStages(i, levelAt(i, step), io[i], br(i), pyr::kStagePost);
         }
         join(main);
     }

@@ -402,94 +402,6 @@ __global__ __launch_bounds__(256) void kCensusLocal(const uint32_t* cen, RefineI
     }
 }
 
-// ---- BT cost + horizontal window sum (sgm_kernels.cu:139-239, :354-379) ------
-// Block: one 64-pixel row segment x 64 labels.  BT costs of the segment and its
-// +-half halo (clamped columns) go to LDS as bytes; each thread then sums the
-// window for 8 labels of one pixel and saturates at SHRT_MAX.
-constexpr int kSegX = 64;
-constexpr int kChunkD = 64;
-
-__global__ __launch_bounds__(256) void kBTRow(const uint2* desc, int16_t* Hs, int W, int H,
-                                              int Dp, int minD, int half, size_t wsStride)
-{
-    extern __shared__ uint8_t s_bt[];
-    const int dc = imin(Dp, kChunkD);
-    const int nzc = Dp / dc;  // label chunks per pair
-    const size_t pr = blockIdx.z / nzc;
-    desc = pairAt(desc, pr, wsStride);
-    Hs = pairAt(Hs, pr, wsStride);
-    const int y = blockIdx.y;
-    const int x0 = blockIdx.x * kSegX;
-    const int dc0 = (blockIdx.z - (int)pr * nzc) * dc;
-    const int ext = kSegX + 2 * half;
-    const uint2* L = desc + (size_t)y * W;
-    const uint2* R = desc + (size_t)W * H + (size_t)y * W;
-    for (int i = threadIdx.x; i < ext * dc; i += 256) {
-        const int o = i / dc;
-        const int dl = i - o * dc;
-        const int xx = clampi(x0 - half + o, 0, W - 1);
-        const int x2 = xx - (dc0 + dl) - minD;
-        s_bt[i] = (x2 >= 0 && x2 < W) ? (uint8_t)btCost(L[xx], R[x2]) : (uint8_t)255;
-    }
-    __syncthreads();
-    const int tpp = dc >> 3;
-    for (int t = threadIdx.x; t < kSegX * tpp; t += 256) {
-        const int px = t / tpp;
-        const int dl0 = (t - px * tpp) * 8;
-        const int x = x0 + px;
-        if (x >= W) continue;
-        int v[8];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) v[i] = 0;
-        for (int k = 0; k <= 2 * half; ++k) {
-            const uint8_t* b = s_bt + (px + k) * dc + dl0;
-#pragma unroll
-            for (int i = 0; i < 8; ++i) v[i] += b[i];
-        }
-#pragma unroll
-        for (int i = 0; i < 8; ++i) v[i] = imin(kMax, v[i]);
-        store8(Hs + ((size_t)y * W + x) * Dp + dc0 + dl0, v);
-    }
-}
-
-// ---- vertical window sum (sgm_kernels.cu:390-416) ----------------------------
-__global__ __launch_bounds__(256) void kBoxV(const int16_t* Hs, int16_t* C, int W, int H, int D,
-                                             int Dp, int half, int minD, int maxD, int geomMask,
-                                             size_t wsStride)
-{
-    Hs = pairAt(Hs, blockIdx.y, wsStride);
-    C = pairAt(C, blockIdx.y, wsStride);
-    const size_t oct = (size_t)blockIdx.x * 256 + threadIdx.x;
-    const size_t nOct = (size_t)W * H * (Dp >> 3);
-    if (oct >= nOct) return;
-    const size_t p = oct / (Dp >> 3);
-    const int d0 = (int)(oct - p * (Dp >> 3)) * 8;
-    const int y = (int)(p / W);
-    const int x = (int)(p - (size_t)y * W);
-    int v[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) v[i] = 0;
-    for (int k = -half; k <= half; ++k) {
-        const int yy = clampi(y + k, 0, H - 1);
-        const uint4 w = *reinterpret_cast<const uint4*>(Hs + ((size_t)yy * W + x) * Dp + d0);
-        v[0] += (int16_t)(w.x & 0xFFFF); v[1] += (int)w.x >> 16;
-        v[2] += (int16_t)(w.y & 0xFFFF); v[3] += (int)w.y >> 16;
-        v[4] += (int16_t)(w.z & 0xFFFF); v[5] += (int)w.z >> 16;
-        v[6] += (int16_t)(w.w & 0xFFFF); v[7] += (int)w.w >> 16;
-    }
-    const int hiReal = imin(maxD, x);
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        v[i] = imin(kMax, v[i]);
-        const int real = minD + d0 + i;
-        // Padding labels, and at coarse pyramid levels geometry-invalid labels,
-        // carry SHRT_MAX (see kCensusCost).
-        if (d0 + i >= D || (geomMask && (real > hiReal || x - real < 0 || x - real >= W))) v[i] = kMax;
-    }
-    storeCost8(C + p * Dp + d0, v);
-}
-
-// ---- BT window cost at refinement levels (sgm_kernels.cu:1075-1110) ----------
 // Packed form of one BT descriptor: (v, v0, v1) of both channels as 16-bit
 // pairs, gradient channel low, intensity channel high.
 struct BTPacked {
@@ -527,6 +439,109 @@ __device__ __forceinline__ int btCostPacked(const BTPacked& l, const BTPacked& r
     return imin(255, (int)(c & 0xFFFFu) + (int)(c >> 18));
 }
 
+// ---- BT cost + horizontal window sum (sgm_kernels.cu:139-239, :354-379) ------
+// Block: one 64-pixel row segment x 64 labels.  BT costs of the segment and its
+// +-half halo (clamped columns) go to LDS as bytes (from the
+// segment's descriptors, unpacked into LDS first); each thread then sums the
+// window for 8 labels of one pixel and saturates at SHRT_MAX.
+constexpr int kSegX = 64;
+constexpr int kChunkD = 64;
+
+__global__ __launch_bounds__(256) void kBTRow(const uint2* desc, int16_t* Hs, int W, int H,
+                                              int Dp, int minD, int half, size_t wsStride)
+{
+    extern __shared__ __attribute__((aligned(16))) uint8_t s_bt[];
+    const int dc = imin(Dp, kChunkD);      // a power of two
+    const int dsh = 31 - __clz(dc);
+    const int nzc = Dp / dc;  // label chunks per pair
+    const size_t pr = blockIdx.z / nzc;
+    desc = pairAt(desc, pr, wsStride);
+    Hs = pairAt(Hs, pr, wsStride);
+    const int y = blockIdx.y;
+    const int x0 = blockIdx.x * kSegX;
+    const int dc0 = (blockIdx.z - (int)pr * nzc) * dc;
+    const int ext = kSegX + 2 * half;
+    const uint2* L = desc + (size_t)y * W;
+    const uint2* R = desc + (size_t)W * H + (size_t)y * W;
+    // Both views' descriptors of the segment, unpacked once into LDS: left
+    // columns xx = clamp(x0 - half + o), right columns x2 = xx - d - minD for
+    // the chunk's labels d (rbase .. rbase + nr - 1).
+    const int btBytes = (ext * dc + 15) & ~15;
+    BTPacked* s_l = reinterpret_cast<BTPacked*>(s_bt + btBytes);
+    BTPacked* s_r = s_l + ext;
+    const int xlo = clampi(x0 - half, 0, W - 1), xhi = clampi(x0 + kSegX - 1 + half, 0, W - 1);
+    const int rbase = xlo - (dc0 + dc - 1) - minD;
+    const int nr = xhi - xlo + dc;  // <= ext + dc - 1
+    for (int o = threadIdx.x; o < ext; o += 256) s_l[o] = btPack(L[clampi(x0 - half + o, 0, W - 1)]);
+    for (int j = threadIdx.x; j < nr; j += 256) {
+        const int x2 = rbase + j;
+        s_r[j] = (x2 >= 0 && x2 < W) ? btPack(R[x2]) : btPoison();
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < ext * dc; i += 256) {
+        const int o = i >> dsh;
+        const int dl = i - (o << dsh);
+        const int xx = clampi(x0 - half + o, 0, W - 1);
+        // An out-of-image right column holds the poison descriptor: cost 255.
+        s_bt[i] = (uint8_t)btCostPacked(s_l[o], s_r[xx - (dc0 + dl) - minD - rbase]);
+    }
+    __syncthreads();
+    const int tpp = dc >> 3;
+    for (int t = threadIdx.x; t < kSegX * tpp; t += 256) {
+        const int px = t / tpp;
+        const int dl0 = (t - px * tpp) * 8;
+        const int x = x0 + px;
+        if (x >= W) continue;
+        int v[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = 0;
+        for (int k = 0; k <= 2 * half; ++k) {
+            const uint8_t* b = s_bt + (px + k) * dc + dl0;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) v[i] += b[i];
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = imin(kMax, v[i]);
+        store8(Hs + ((size_t)y * W + x) * Dp + dc0 + dl0, v);
+    }
+}
+__global__ __launch_bounds__(256) void kBoxV(const int16_t* Hs, int16_t* C, int W, int H, int D,
+                                             int Dp, int half, int minD, int maxD, int geomMask,
+                                             size_t wsStride)
+{
+    Hs = pairAt(Hs, blockIdx.y, wsStride);
+    C = pairAt(C, blockIdx.y, wsStride);
+    const size_t oct = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t nOct = (size_t)W * H * (Dp >> 3);
+    if (oct >= nOct) return;
+    const size_t p = oct / (Dp >> 3);
+    const int d0 = (int)(oct - p * (Dp >> 3)) * 8;
+    const int y = (int)(p / W);
+    const int x = (int)(p - (size_t)y * W);
+    int v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = 0;
+    for (int k = -half; k <= half; ++k) {
+        const int yy = clampi(y + k, 0, H - 1);
+        const uint4 w = *reinterpret_cast<const uint4*>(Hs + ((size_t)yy * W + x) * Dp + d0);
+        v[0] += (int16_t)(w.x & 0xFFFF); v[1] += (int)w.x >> 16;
+        v[2] += (int16_t)(w.y & 0xFFFF); v[3] += (int)w.y >> 16;
+        v[4] += (int16_t)(w.z & 0xFFFF); v[5] += (int)w.z >> 16;
+        v[6] += (int16_t)(w.w & 0xFFFF); v[7] += (int)w.w >> 16;
+    }
+    const int hiReal = imin(maxD, x);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        v[i] = imin(kMax, v[i]);
+        const int real = minD + d0 + i;
+        // Padding labels, and at coarse pyramid levels geometry-invalid labels,
+        // carry SHRT_MAX (see kCensusCost).
+        if (d0 + i >= D || (geomMask && (real > hiReal || x - real < 0 || x - real >= W))) v[i] = kMax;
+    }
+    storeCost8(C + p * Dp + d0, v);
+}
+
+// ---- BT window cost at refinement levels (sgm_kernels.cu:1075-1110) ----------
 // Window sums of 8 consecutive labels (real disparities r0 .. r0 + 7) of a
 // pixel whose window columns x - HALF .. x + HALF all lie inside the image:
 // per window row, the 2 HALF + 1 left and 2 HALF + 8 right descriptors the
@@ -987,7 +1002,8 @@ hipError_t launchCost(const LevelCall& c, Workspace& ws, hipStream_t s)
     }
     const int dc = c.Dp < kChunkD ? c.Dp : kChunkD;
     dim3 gridH((c.W + kSegX - 1) / kSegX, c.H, (c.Dp / dc) * c.batch.n);
-    const size_t lds = (size_t)(kSegX + 2 * half) * dc;
+    const int ext = kSegX + 2 * half;
+    const size_t lds = (size_t)((ext * dc + 15) & ~15) + (size_t)(2 * ext + dc) * sizeof(BTPacked);
     kBTRow<<<gridH, 256, lds, s>>>(ws.desc, ws.Alr, c.W, c.H, c.Dp, c.minD, half, c.batch.ws);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;

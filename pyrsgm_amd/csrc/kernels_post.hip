@@ -35,22 +35,20 @@ struct PostArgs {
 };
 
 // The two right-view keys the LR check of disparity d at (x, y) reads (columns
-// x - (d >> 4) and x - ((d + 15) >> 4)); none when the check is off, d is
-// invalid or the column is outside the image.  Loads only, so that a thread
-// can have the keys of several pixels in flight at once.
+// x - (d >> 4) and x - ((d + 15) >> 4)).  Loads only, so that a thread can have
+// the keys of several pixels in flight at once: both keys are read from
+// clamped in-image cells whatever d is, with no branch between the loads (a
+// load under a branch is waited for where the branch joins).  lrDecide uses a
+// key only when d is valid and its column is inside the image.
 struct LrProbe {
     uint32_t ka, kb;
 };
 
-__device__ __forceinline__ LrProbe lrLoad(const PostArgs& a, int x, int y, int d, int invalid)
+__device__ __forceinline__ LrProbe lrLoad(const PostArgs& a, int x, int y, int d)
 {
-    LrProbe pr{0xFFFFFFFFu, 0xFFFFFFFFu};
-    if (!a.lrcheck || d == invalid) return pr;
     const int xa = x - (d >> 4), xb = x - ((d + 15) >> 4);
-    const uint32_t* row = a.keys + (size_t)y * a.W;
-    if (xa >= 0 && xa < a.W) pr.ka = row[xa];
-    if (xb >= 0 && xb < a.W) pr.kb = row[xb];
-    return pr;
+    const uint32_t* row = a.keys + (size_t)imin(imax(y, 0), a.H - 1) * a.W;
+    return {row[imin(imax(xa, 0), a.W - 1)], row[imin(imax(xb, 0), a.W - 1)]};
 }
 
 __device__ __forceinline__ int disp2Of(uint32_t k, int x2, int invalid)
@@ -140,27 +138,36 @@ __global__ __launch_bounds__(256) void kPost(PostArgs args)
             hx[k] = bx + tx - 1;
             hy[k] = by + ty - 1;
             const bool in = i < HALO && hx[k] >= 0 && hx[k] < a.W && hy[k] >= 0 && hy[k] < a.H;
-            hd[k] = in ? a.disp[(size_t)hy[k] * a.W + hx[k]] : invalid;
+            // Loaded from a clamped cell unconditionally (see lrLoad).
+            const int v = a.disp[(size_t)imin(imax(hy[k], 0), a.H - 1) * a.W + imin(imax(hx[k], 0), a.W - 1)];
+            hd[k] = in ? v : invalid;
             if (!in) hx[k] = -1;
         }
-        LrProbe pb[NH];
+        LrProbe pb[NH] = {};
+        if (a.lrcheck) {
 #pragma unroll
-        for (int k = 0; k < NH; ++k)
-            pb[k] = hx[k] >= 0 ? lrLoad(a, hx[k], hy[k], hd[k], invalid) : LrProbe{0xFFFFFFFFu, 0xFFFFFFFFu};
+            for (int k = 0; k < NH; ++k) pb[k] = lrLoad(a, hx[k], hy[k], hd[k]);
+        }
+        // Out-of-image cells hold `invalid` (d = invalid passes lrDecide
+        // unchanged); the median reads in-image cells only.
+        int ck[NH];
+#pragma unroll
+        for (int k = 0; k < NH; ++k) ck[k] = lrDecide(a, hx[k], hd[k], pb[k], invalid);
 #pragma unroll
         for (int k = 0; k < NH; ++k) {
             const int i = threadIdx.x + k * 256;
             const int ty = i / HW, tx = i - ty * HW;
-            if (hx[k] >= 0) s_d[ty][tx] = (int16_t)lrDecide(a, hx[k], hd[k], pb[k], invalid);
+            if (i < HALO) s_d[ty][tx] = (int16_t)ck[k];
         }
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < kPostR; ++r) chk[r] = s_d[ly0 + 4 * r + 1][lx + 1];
     } else {
-        LrProbe pb[kPostR];
+        LrProbe pb[kPostR] = {};
+        if (a.lrcheck) {
 #pragma unroll
-        for (int r = 0; r < kPostR; ++r)
-            pb[r] = lrLoad(a, xc, imin(by + ly0 + 4 * r, a.H - 1), raw[r], invalid);
+            for (int r = 0; r < kPostR; ++r) pb[r] = lrLoad(a, xc, imin(by + ly0 + 4 * r, a.H - 1), raw[r]);
+        }
 #pragma unroll
         for (int r = 0; r < kPostR; ++r) chk[r] = lrDecide(a, xc, raw[r], pb[r], invalid);
     }

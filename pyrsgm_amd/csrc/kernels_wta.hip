@@ -61,6 +61,9 @@ __device__ __forceinline__ void wtaBlock(const PathArgs& a, size_t pr, size_t bx
     const size_t pc = ok ? p : P - 1;
     const size_t off = pc * a.Dp + (size_t)sub * 16;
     const int np = a.numPaths;
+    // The pixel's label word is loaded with the volumes (pc is in bounds), so
+    // that the wave waits on memory once, not again after the label search.
+    const int lwPre = MODE == kRefine ? labels[pc] : 0;
 
     const int d0 = sub * 16;
     uint32_t kmin;
@@ -226,7 +229,7 @@ __device__ __forceinline__ void wtaBlock(const PathArgs& a, size_t pr, size_t bx
     } else {
         int base = a.minD, valid = 1;
         if constexpr (MODE == kRefine) {
-            const int lw = labels[p];
+            const int lw = lwPre;
             base = (int)(int16_t)(lw & 0xFFFF);
             valid = (lw >> 16) & 1;
         }

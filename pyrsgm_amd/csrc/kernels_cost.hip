@@ -377,6 +377,13 @@ __global__ __launch_bounds__(256) void kCensusLocal(const uint32_t* cen, RefineI
     }
     if (x >= W) return;
     const int lo = imax(minD, x - W + 1), hi = imin(maxD, x);
+    // Byte volumes from staged words over a full label range (uniform): the
+    // staged index x - (base + d) - xb of every label lies in [0, span),
+    // because refineFrom keeps base in [minD, maxD - D + 1] and d < D.  The
+    // eight reads of a thread then share one address (immediate offsets), the
+    // byte is 255 - popc(l ^ r) = 223 + popc(~(l ^ r)), and the label range
+    // [lo, hi] and the parent flag are applied as one byte mask.
+    const bool direct = STAGED && NARROW && Dp == D && maxD - minD + 1 >= D;
 #pragma unroll
     for (int rr = 0; rr < ROWS; ++rr) {
         const int y = y0 + rr;
@@ -385,6 +392,20 @@ __global__ __launch_bounds__(256) void kCensusLocal(const uint32_t* cen, RefineI
         refineFrom(r, raw[rr], cf[rr], x, D, minD, maxD, base, valid);
         const size_t p = (size_t)y * W + x;
         if (d0 == 0) labels[p] = labelWord(base, valid, im[rr]);
+        if (STAGED && NARROW && direct) {
+            const uint32_t* const Sc = s_r + rr * span + (x - base - d0 - xb);
+            uint32_t w[2] = {0u, 0u};
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                w[i >> 2] |= (223u + (uint32_t)__popc(~(cl[rr] ^ Sc[-i]))) << (8 * (i & 3));
+            // Labels d0 + i with lo <= base + d0 + i <= hi, as bits, then bytes.
+            const int ilo = imax(lo - base - d0, 0), ihi = imin(hi - base - d0, 7);
+            const uint32_t ok = (valid && ilo <= ihi) ? ((0xFFu >> (7 - (ihi - ilo))) << ilo) : 0u;
+            auto bytes = [](uint32_t m) { return ((m * 0x00204081u) & 0x01010101u) * 0xFFu; };
+            *reinterpret_cast<uint2*>(reinterpret_cast<uint8_t*>(C) + p * Dp + d0) =
+                make_uint2(w[0] & bytes(ok & 0xFu), w[1] & bytes(ok >> 4));
+            continue;
+        }
         const uint32_t* R = cen + (size_t)W * H + (size_t)y * W;
         const uint32_t* S = s_r + rr * span;
         int v[8];

@@ -11,6 +11,8 @@
 // strict LR check :855-889 (pyramid), median :891-920 (in-bounds neighbours,
 // sorted[n/2]), confidence clear :922-931, float conversion :1530-1551, output
 // routing :1837-1858 / :2053-2069.
+#include <type_traits>
+
 #include "device_common.h"
 #include "internal.h"
 
@@ -172,11 +174,15 @@ __global__ __launch_bounds__(256) void kPost(PostArgs args)
         for (int r = 0; r < kPostR; ++r) chk[r] = lrDecide(a, xc, raw[r], pb[r], invalid);
     }
     if (x >= a.W) return;
+    // IN (compile time): the block and its one-pixel ring lie inside the image,
+    // so every pixel has nine in-image neighbours and no bound is tested.
+    auto finish = [&](auto inC) {
+    constexpr bool IN = decltype(inC)::value;
 #pragma unroll
     for (int r = 0; r < kPostR; ++r) {
         const int ly = ly0 + 4 * r;
         const int y = by + ly;
-        if (y >= a.H) break;
+        if (!IN && y >= a.H) break;
         float cf = (chk[r] == raw[r]) ? cf0[r] : 0.0f;
         int fin = chk[r];
         if (a.median) {
@@ -186,7 +192,7 @@ __global__ __launch_bounds__(256) void kPost(PostArgs args)
             for (int dy = -1; dy <= 1; ++dy)
 #pragma unroll
                 for (int dx = -1; dx <= 1; ++dx)
-                    n += x + dx >= 0 && x + dx < a.W && y + dy >= 0 && y + dy < a.H;
+                    n += IN || (x + dx >= 0 && x + dx < a.W && y + dy >= 0 && y + dy < a.H);
             const int nLow = 4 - n / 2;
             int seenOut = 0;
             int idx = 0;
@@ -194,7 +200,7 @@ __global__ __launch_bounds__(256) void kPost(PostArgs args)
             for (int dy = -1; dy <= 1; ++dy)
 #pragma unroll
                 for (int dx = -1; dx <= 1; ++dx) {
-                    const bool in = x + dx >= 0 && x + dx < a.W && y + dy >= 0 && y + dy < a.H;
+                    const bool in = IN || (x + dx >= 0 && x + dx < a.W && y + dy >= 0 && y + dy < a.H);
                     int val;
                     if (in) {
                         val = s_d[ly + 1 + dy][lx + 1 + dx];
@@ -217,6 +223,11 @@ __global__ __launch_bounds__(256) void kPost(PostArgs args)
             *reinterpret_cast<float*>((char*)a.outc + (size_t)y * a.sc + (size_t)x * 4) = cf;
         }
     }
+    };
+    if (bx >= 1 && bx + kPostW <= a.W - 1 && by >= 1 && by + kPostH <= a.H - 1)
+        finish(std::true_type{});
+    else
+        finish(std::false_type{});
 }
 
 } // namespace

@@ -240,6 +240,36 @@ __global__ __launch_bounds__(256) void kCensusCost(const uint32_t* cen, int16_t*
     if (x >= W) return;
     const uint32_t cl = L[x];
     const int hiReal = imin(maxD, x);
+    if constexpr (NARROW) {
+        // Byte form with label masks: the staged index (x - x0) + Dp - 1 - d is
+        // always inside the staged row, so the eight reads share one address;
+        // the byte is 255 - popc(l ^ r) = 223 + popc(~(l ^ r)).  Label i of the
+        // thread (d = d0 + i) exists if d < D; its right column is in the image
+        // if x - W + 1 <= minD + d <= x; at a coarse level it is also invalid
+        // above hiReal.  Out-of-image columns cost 24 (byte 231) at single
+        // scale and are invalid (byte 0) at coarse levels.
+        const uint32_t* const Sc = s_r + (x - x0) + Dp - 1 - d0;
+        uint32_t w[2] = {0u, 0u};
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            w[i >> 2] |= (223u + (uint32_t)__popc(~(cl ^ Sc[-i]))) << (8 * (i & 3));
+        auto bits = [](int lo, int hi) {  // bits lo..hi of 0..7 (none if empty)
+            lo = imax(lo, 0);
+            hi = imin(hi, 7);
+            return lo <= hi ? (0xFFu >> (7 - (hi - lo))) << lo : 0u;
+        };
+        auto bytes = [](uint32_t m) { return ((m * 0x00204081u) & 0x01010101u) * 0xFFu; };
+        const uint32_t exist = bits(0, D - 1 - d0);
+        const uint32_t in = bits(x - W + 1 - minD - d0, x - minD - d0);
+        uint32_t keep = exist & in, out = 0u;
+        if (geomMask) keep &= bits(0, hiReal - minD - d0);
+        else out = exist & ~in;
+        const uint32_t k0 = bytes(keep & 0xFu), k1 = bytes(keep >> 4);
+        const uint32_t o0 = bytes(out & 0xFu) & 0xE7E7E7E7u, o1 = bytes(out >> 4) & 0xE7E7E7E7u;
+        *reinterpret_cast<uint2*>(reinterpret_cast<uint8_t*>(C) + ((size_t)y * W + x) * Dp + d0) =
+            make_uint2((w[0] & k0) | o0, (w[1] & k1) | o1);
+        return;
+    }
     int v[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) {

@@ -35,15 +35,24 @@ def pair(w, h, maxd, seed):
 
 
 # --------------------------------------------------------------- pyrDown
-@pytest.mark.parametrize("w,h", [(64, 48), (64, 33), (97, 48), (97, 33), (2, 2), (3, 5)])
-def test_pyrdown_kat(w, h):
-    xs, ys = np.meshgrid(np.arange(w), np.arange(h))
-    src = ((xs * 13 + ys * 29 + (xs * ys) // 7) & 0xFF).astype(np.uint8)
-    want = O.pyrdown(src)
-    S = U.Img.upload(src)
+# Sizes from 643 x 301 up have interior blocks (no reflection) beside border
+# blocks; off = 1 starts the source rows one byte past a 4-byte boundary.
+@pytest.mark.parametrize("w,h,off", [(64, 48, 0), (64, 33, 0), (97, 48, 0), (97, 33, 0), (2, 2, 0),
+                                     (3, 5, 0), (643, 301, 0), (1920, 1080, 0), (643, 301, 1)])
+def test_pyrdown_kat(w, h, off):
+    xs, ys = np.meshgrid(np.arange(w + off), np.arange(h))
+    full = ((xs * 13 + ys * 29 + (xs * ys) // 7) & 0xFF).astype(np.uint8)
+    src = full[:, off:]
+    want = O.pyrdown(np.ascontiguousarray(src))
+    torch = __import__("torch")
+    pitch = (w + off + 63) // 64 * 64  # 4-byte aligned rows
+    buf = np.zeros((h, pitch), np.uint8)
+    buf[:, :w + off] = full
+    S = torch.from_numpy(buf).cuda()
     dh, dw = want.shape
-    Dd = U.Img(dh, dw, __import__("torch").uint8, 77)
-    assert P.pyrdown_u8(S.ptr, S.pitch, w, h, Dd.ptr, Dd.pitch, dw, dh, U.stream()) == 0
+    Dd = U.Img(dh, dw, torch.uint8, 77)
+    print("case: pyrdown", w, h, "offset", off, flush=True)
+    assert P.pyrdown_u8(S.data_ptr() + off, pitch, w, h, Dd.ptr, Dd.pitch, dw, dh, U.stream()) == 0
     U.sync()
     U.assert_same("pyrdown", Dd.numpy(), want)
 

@@ -15,7 +15,9 @@ from dataclasses import dataclass
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+# PYRSGM_ORACLE_LIB points the tests at another build of the same source (a
+# copy with a planted mistake, to check that the pins catch it).
+_LIB_PATH = os.environ.get("PYRSGM_ORACLE_LIB") or os.path.join(_HERE, "_build", "liboracle.so")
 _lib = None
 
 COST_BT, COST_CENSUS = 0, 1

@@ -289,6 +289,44 @@ def invalid_disparity16(p: Params) -> int:
     return v - 0x10000 if v >= 0x8000 else v
 
 
+def _check(t, name, dtype, shape):
+    """Raises ValueError unless `t` is a device tensor of `dtype` whose shape is
+    `shape` (None entries match any size) with contiguous rows (last stride 1)
+    and a non-negative row pitch.  The kernels address rows by pointer and
+    pitch only, so a smaller, strided or differently typed tensor would be
+    read or written out of bounds."""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"pyrsgm: {name} must be a torch tensor, got {type(t).__name__}")
+    if not t.is_cuda:
+        raise ValueError(f"pyrsgm: {name} must be a device tensor")
+    if t.dtype != dtype:
+        raise ValueError(f"pyrsgm: {name} must be {dtype}, got {t.dtype}")
+    if t.dim() != len(shape) or any(s is not None and s != n for s, n in zip(shape, t.shape)):
+        want = ", ".join("N" if s is None else str(s) for s in shape)
+        raise ValueError(f"pyrsgm: {name} must have shape [{want}], got {list(t.shape)}")
+    if t.stride(-1) != 1 or any(t.stride(i) < 0 for i in range(t.dim())):
+        raise ValueError(f"pyrsgm: {name} rows must be contiguous (last stride 1), "
+                         f"got strides {t.stride()}")
+    for i in range(t.dim() - 1):
+        # Rows (and pairs) must not overlap: each outer stride spans the inner extent.
+        if t.shape[i] > 1 and t.stride(i) < t.shape[i + 1] * t.stride(i + 1):
+            raise ValueError(f"pyrsgm: {name} has overlapping rows or pairs (strides {t.stride()}, "
+                             f"shape {list(t.shape)})")
+
+
+# Tensor roles and their element types.
+_ROLE_DTYPES = {"left": "uint8", "right": "uint8", "disp16": "int16", "disp32": "float32",
+                "conf": "float32"}
+
+
+def _checked(role, t, shape):
+    if t is not None:
+        import torch
+        _check(t, role, getattr(torch, _ROLE_DTYPES[role]), shape)
+    return t
+
+
 def _ptr(t):
     if t is None:
         return None, 0
@@ -332,6 +370,13 @@ class StereoMatcher:
         """Torch-tensor convenience: 2-D device tensors (uint8 in, int16/float32 out)."""
         import torch
         s = stream if stream is not None else torch.cuda.current_stream()
+        hw = (self._hgt, self._w)
+        for role, t in (("left", left), ("right", right)):
+            if t is None:
+                raise ValueError(f"pyrsgm: {role} image is required")
+            _checked(role, t, hw)
+        for role, t in (("disp16", disp16), ("disp32", disp32), ("conf", conf)):
+            _checked(role, t, hw)
         l, lp = _ptr(left)
         r, rp = _ptr(right)
         d16, p16 = _ptr(disp16)
@@ -410,11 +455,16 @@ class BatchMatcher:
         count = self.batch if count is None else count
         if not 1 <= count <= self.batch:
             raise ValueError(f"pyrsgm: pair count {count} outside [1, {self.batch}]")
-        for t in (left, right, disp16, disp32, conf):
-            if t is not None and (t.dim() != 3 or t.shape[0] < count or t.shape[0] > self.batch or
-                                  t.shape[1] != self.height or t.shape[2] != self.width):
-                raise ValueError(f"pyrsgm: expected [{count}..{self.batch}, {self.height}, "
-                                 f"{self.width}] tensors, got {tuple(t.shape)}")
+        for role, t in (("left", left), ("right", right), ("disp16", disp16),
+                        ("disp32", disp32), ("conf", conf)):
+            if t is None:
+                if role in ("left", "right"):
+                    raise ValueError(f"pyrsgm: {role} images are required")
+                continue
+            _checked(role, t, (None, self.height, self.width))
+            if t.shape[0] < count or t.shape[0] > self.batch:
+                raise ValueError(f"pyrsgm: {role} holds {t.shape[0]} pairs; expected "
+                                 f"{count}..{self.batch}")
         out = []
         for t in (left, right, disp16, disp32, conf):
             out += list(_bptr(t))
@@ -531,9 +581,15 @@ class MixedBatch:
         arr = (_PairIOC * len(pairs))()
         for i, ((w, h, _), tensors) in enumerate(zip(self.specs, pairs)):
             vals = []
-            for t in tensors:
-                if t is not None and (t.dim() != 2 or t.shape[0] != h or t.shape[1] != w):
-                    raise ValueError(f"pyrsgm: pair {i}: expected [{h}, {w}] tensors, got {tuple(t.shape)}")
+            if len(tensors) != 5:
+                raise ValueError(f"pyrsgm: pair {i}: expected (left, right, disp16, disp32, conf)")
+            for role, t in zip(("left", "right", "disp16", "disp32", "conf"), tensors):
+                if t is None and role in ("left", "right"):
+                    raise ValueError(f"pyrsgm: pair {i}: {role} image is required")
+                try:
+                    _checked(role, t, (h, w))
+                except ValueError as e:
+                    raise ValueError(f"pyrsgm: pair {i}: {e}") from None
                 vals += list(_ptr(t))
             for (name, _), v in zip(_PairIOC._fields_, vals):
                 setattr(arr[i], name, v)
@@ -627,14 +683,16 @@ class Server:
     def submit(self, left, right, params: Params, disp16=None, disp32=None, conf=None,
                arrival_ns: int = 0) -> int:
         """Queue one pair; returns its ticket.  arrival_ns (steady clock) defaults to now."""
+        if left is None or right is None:
+            raise ValueError("pyrsgm: left and right images are required")
+        _checked("left", left, (None, None))
         h, w = left.shape
         req = _ServeRequestC()
         req.width, req.height = w, h
         for name, t in (("left", left), ("right", right), ("disp16", disp16),
                         ("disp32", disp32), ("conf", conf)):
+            _checked(name, t, (h, w))
             ptr, pitch = _ptr(t)
-            if t is not None and tuple(t.shape) != (h, w):
-                raise ValueError(f"pyrsgm: {name} has shape {tuple(t.shape)}, expected {(h, w)}")
             setattr(req, name, ptr)
             setattr(req, name + "_pitch", pitch)
         err = C.create_string_buffer(512)

@@ -5,7 +5,9 @@
 // owns the lanes and is the only thread that issues device work.  It reaps
 // finished launches (event queries), admits pending requests to lanes and
 // launches every idle lane that has requests, releasing the mutex while it
-// enqueues device work.
+// enqueues device work.  With nothing to do it sleeps on a doorbell that a
+// submit, the destructor, or a lane's completion rings; a lane's completion
+// rings it from a host function enqueued behind the lane's work.
 #include "pyrsgm/pyrsgm_serve.h"
 
 #include <algorithm>
@@ -15,6 +17,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <unordered_map>
@@ -40,6 +43,46 @@ std::vector<int32_t> makeKey(int w, int h, int outputs, const Params& p)
             p.lr_check, p.median_filter, p.num_paths, p.pre_filter_cap, p.pyramid_levels,
             p.pyramid_coarse_disparities, p.pyramid_max_disparity, f(p.pyramid_min_confidence),
             p.adaptive_p2, f(p.p2_alpha), f(p.p2_beta), f(p.p2_lower_mult), f(p.p2_upper_mult)};
+}
+
+// Wake-up counter of the scheduler.  It has a mutex of its own, which nothing
+// holds across a HIP call: the host functions that ring it run on the HIP
+// runtime's callback thread and must never wait for the scheduler's mutex
+// (a hipFree under that mutex waits for the streams, whose host functions
+// would wait for the mutex).
+struct Doorbell {
+    std::mutex m;
+    std::condition_variable cv;
+    uint64_t rings = 0;
+
+    void ring()
+    {
+        {
+            std::lock_guard<std::mutex> lk(m);
+            ++rings;
+        }
+        cv.notify_all();
+    }
+    uint64_t count()
+    {
+        std::lock_guard<std::mutex> lk(m);
+        return rings;
+    }
+    // Returns once the counter has moved past `seen`.
+    void waitPast(uint64_t seen)
+    {
+        std::unique_lock<std::mutex> lk(m);
+        cv.wait(lk, [&] { return rings != seen; });
+    }
+};
+
+// Host function enqueued behind a lane's work: rings the doorbell and drops
+// the reference it was given (the doorbell outlives every pending ring).
+void ringFromStream(void* user)
+{
+    auto* ref = static_cast<std::shared_ptr<Doorbell>*>(user);
+    (*ref)->ring();
+    delete ref;
 }
 
 struct Req {
@@ -146,7 +189,8 @@ struct Lane {
               "scatter");
     }
 
-    void launch(const std::vector<Req*>& batch, bool useGraphs)
+    void launch(const std::vector<Req*>& batch, bool useGraphs,
+                const std::shared_ptr<Doorbell>& bell)
     {
         const int n = (int)batch.size();
         const size_t sL = inL.pitch * (size_t)h, sR = inR.pitch * (size_t)h;
@@ -195,6 +239,12 @@ struct Lane {
             enqueue(n);
         }
         check(hipEventRecord(done, stream), "hipEventRecord");
+        auto* ref = new std::shared_ptr<Doorbell>(bell);
+        const hipError_t e = hipLaunchHostFunc(stream, ringFromStream, ref);
+        if (e != hipSuccess) {
+            delete ref;
+            check(e, "hipLaunchHostFunc");
+        }
     }
 };
 
@@ -204,7 +254,8 @@ struct Server::Impl {
     ServeOptions opt;
     int device = 0;
     mutable std::mutex mu;
-    std::condition_variable cvWork, cvDone;
+    std::condition_variable cvDone;
+    std::shared_ptr<Doorbell> bell = std::make_shared<Doorbell>();
     std::deque<Req*> pending;
     std::unordered_map<uint64_t, Req*> inflight;   // submitted, not finished
     std::unordered_map<uint64_t, Req*> finished;   // finished, not yet retrieved
@@ -240,7 +291,7 @@ struct Server::Impl {
             std::lock_guard<std::mutex> lk(mu);
             stopping = true;
         }
-        cvWork.notify_all();
+        bell->ring();
         worker.join();
         lanes.clear();
         for (auto& kv : finished) delete kv.second;
@@ -293,7 +344,7 @@ struct Server::Impl {
         inflight[q->ticket] = q.get();
         pending.push_back(q.get());
         const uint64_t t = q.release()->ticket;
-        cvWork.notify_one();
+        bell->ring();
         return t;
     }
 
@@ -443,6 +494,9 @@ struct Server::Impl {
         (void)hipSetDevice(device);
         std::unique_lock<std::mutex> lk(mu);
         for (;;) {
+            // Rings after this snapshot (a submit, a finished lane) end the
+            // sleep below, so none is lost between the checks and the wait.
+            const uint64_t seen = bell->count();
             bool progress = reap();
             auto batches = admit();
             if (!batches.empty()) {
@@ -461,7 +515,7 @@ struct Server::Impl {
                 std::vector<Lane*> bad;
                 for (auto& b : batches) {
                     try {
-                        b.first->launch(b.second, opt.use_graphs);
+                        b.first->launch(b.second, opt.use_graphs, bell);
                     } catch (const std::exception&) {
                         bad.push_back(b.first);
                     }
@@ -479,13 +533,10 @@ struct Server::Impl {
             for (auto& lp : lanes) anyBusy |= lp->busy;
             if (stopping && pending.empty() && !anyBusy) break;
             if (!progress) {
-                if (anyBusy) {
-                    lk.unlock();
-                    std::this_thread::yield();
-                    lk.lock();
-                } else {
-                    cvWork.wait(lk, [&] { return stopping || !pending.empty(); });
-                }
+                // Busy lanes ring on completion; submits and stop ring too.
+                lk.unlock();
+                bell->waitPast(seen);
+                lk.lock();
             }
         }
     }
@@ -524,6 +575,7 @@ bool Server::poll(uint64_t ticket, ServeResult* result)
     if (result) Impl::fill(result, *it->second);
     delete it->second;
     impl_->finished.erase(it);
+    impl_->cvDone.notify_all();  // a wait() on this ticket ends
     return true;
 }
 
@@ -532,12 +584,20 @@ ServeResult Server::wait(uint64_t ticket)
     std::unique_lock<std::mutex> lk(impl_->mu);
     if (!impl_->finished.count(ticket) && !impl_->inflight.count(ticket))
         throw std::invalid_argument("pyrsgm: unknown ticket");
-    impl_->cvDone.wait(lk, [&] { return impl_->finished.count(ticket) != 0; });
+    // A ticket leaves `inflight` only into `finished`, and leaves `finished`
+    // when some caller (poll, takeResults, another wait) retrieves it: the
+    // wait ends in either case.
+    impl_->cvDone.wait(lk, [&] {
+        return impl_->finished.count(ticket) != 0 || impl_->inflight.count(ticket) == 0;
+    });
     auto it = impl_->finished.find(ticket);
+    if (it == impl_->finished.end())
+        throw std::invalid_argument("pyrsgm: ticket was retrieved by another call");
     ServeResult r;
     Impl::fill(&r, *it->second);
     delete it->second;
     impl_->finished.erase(it);
+    impl_->cvDone.notify_all();
     return r;
 }
 
@@ -556,6 +616,7 @@ size_t Server::takeResults(ServeResult* out, size_t max)
         delete it->second;
         it = impl_->finished.erase(it);
     }
+    if (n) impl_->cvDone.notify_all();
     return n;
 }
 

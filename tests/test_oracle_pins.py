@@ -413,7 +413,7 @@ def test_refine_labels_hand_cases():
 
 
 def test_level_geometry_reference_case():
-    """tests/matcher_test.cpp:404-418: 961x541, D=128, 3 extra levels."""
+    """tests/matcher_test.cpp:266-281: 961x541, D=128, 3 extra levels."""
     import pyrsgm_amd as P
     p = P.validate_params(P.Params(num_disparities=128, pyramid_levels=3))
     info = O.level_geometry(961, 541, p)
@@ -452,7 +452,7 @@ def disp_stats(d32, gt, margin):
 @pytest.mark.parametrize("levels", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("census", [False, True])
 def test_reference_acceptance_thresholds(levels, census):
-    """tests/matcher_test.cpp:340-383 run on the oracle."""
+    """tests/matcher_test.cpp:198-247 run on the oracle."""
     import pyrsgm_amd as P
     w, h = 640, 480
     p = P.validate_params(P.Params(num_disparities=64, pyramid_levels=levels,
@@ -494,3 +494,306 @@ def test_reference_pyramid_smoke_chain():
         tol = 2.0 if i == 4 else 1.5
         assert v.size > 0 and (np.abs(v - d) <= tol).sum() >= v.size / 2
         parent = out
+
+
+# ------------------------------------- BT gradient: closed form + ramp
+def bt_grad_numpy(img, cap):
+    """sgm_kernels.cu:94-118 as array arithmetic over an edge-padded image:
+    2*(row[x+1]-row[x-1]) + the same difference one row up and one row down,
+    clipped to +-ftzero (ftzero = max(cap, 15) | 1), offset by ftzero and
+    stored as a byte (so it wraps above 255)."""
+    ft = max(cap, 15) | 1
+    p = np.pad(img.astype(np.int64), 1, mode="edge")
+    dx = p[:, 2:] - p[:, :-2]                    # [h + 2, w]
+    g = 2 * dx[1:-1] + dx[:-2] + dx[2:]
+    return ((np.clip(g, -ft, ft) + ft) & 0xFF).astype(np.uint8)
+
+
+@pytest.mark.parametrize("cap", [1, 15, 31, 63, 127, 128, 200])
+def test_bt_grad_against_numpy(cap):
+    img = rng(20 + cap).integers(0, 256, (19, 33), dtype=np.uint8)
+    np.testing.assert_array_equal(O.bt_grad(img, cap), bt_grad_numpy(img, cap))
+
+
+def test_bt_grad_ramp_closed_form():
+    """Interior of I = a*x: every horizontal difference is 2a, so the
+    gradient is 2*2a + 2a + 2a = 8a, offset by ftzero = 63 (no clipping for
+    a <= 7); the left and right border columns see one-sided differences."""
+    for a in range(0, 8):
+        img = np.tile((np.arange(30) * a).astype(np.uint8), (6, 1))
+        g = O.bt_grad(img, 63).astype(int)
+        assert (g[:, 1:-1] == 63 + 8 * a).all(), a
+        assert (g[:, 0] == 63 + 4 * a).all() and (g[:, -1] == 63 + 4 * a).all()
+
+
+# ------------------------------ refinement costs: slices of full volumes
+def geom_valid(base, valid, D, minD, maxD):
+    """Label (y, x, d) of a refinement level exists iff its pixel has a
+    parent, and the real disparity base + d is inside [minD, maxD] and
+    projects into the right image (sgm_kernels.cu:1045-1054)."""
+    h, w = base.shape
+    real = base[:, :, None].astype(np.int64) + np.arange(D)[None, None, :]
+    x = np.arange(w)[None, :, None]
+    ok = (valid[:, :, None] != 0) & (real >= minD) & (real <= maxD) & (x - real >= 0) & (x - real < w)
+    return ok, real
+
+
+def local_case(seed, h=9, w=57, D=16, maxD=40, minD=0):
+    r = rng(seed)
+    left = r.integers(0, 256, (h, w), dtype=np.uint8)
+    right = r.integers(0, 256, (h, w), dtype=np.uint8)
+    base = np.clip(r.integers(-4, 5, (h, w)).cumsum(axis=1) + minD + 6, minD - 3,
+                   maxD - 2).astype(np.int16)
+    valid = (r.random((h, w)) > 0.15).astype(np.uint8)
+    return left, right, base, valid
+
+
+@pytest.mark.parametrize("minD,maxD,D", [(0, 40, 16), (0, 30, 32), (-5, 25, 16), (3, 44, 16)])
+def test_census_cost_local_is_full_volume_slice(minD, maxD, D):
+    """A refinement census cost is the full-range census cost volume read at
+    the label's real disparity (sgm_kernels.cu:1112-1133 against :301-327);
+    labels that do not exist are SHRT_MAX."""
+    left, right, base, valid = local_case(31 + D - minD, minD=minD, maxD=maxD, D=D)
+    cl, cr = O.census(left), O.census(right)
+    full = O.census_cost(cl, cr, maxD - minD + 1, minD)          # index k <-> real minD + k
+    ok, real = geom_valid(base, valid, D, minD, maxD)
+    k = np.clip(real - minD, 0, maxD - minD)
+    want = np.where(ok, np.take_along_axis(full, k, axis=2), MAX)
+    got = O.census_cost_local(cl, cr, base, valid, D, maxD, minD)
+    np.testing.assert_array_equal(got, want)
+    assert (got[ok] <= 24).all() and (got[~ok] == MAX).all()
+
+
+@pytest.mark.parametrize("block,cap,minD", [(1, 63, 0), (3, 63, 0), (5, 63, 0), (7, 63, 0),
+                                            (5, 200, 0), (3, 63, -4), (5, 31, 2)])
+def test_bt_window_cost_local_is_box_of_full_volume(block, cap, minD):
+    """A refinement BT window cost is the clamped block x block window sum of
+    the full-range BT volume at the label's real disparity: the two-pass box
+    filter of bt_cost, sliced at base + d (sgm_kernels.cu:1075-1110 against
+    :137-239 and :354-416; no saturation below 49 * 255)."""
+    maxD = 40
+    left, right, base, valid = local_case(41 + block + cap, minD=minD, maxD=maxD)
+    D = 16
+    full = O.box(O.bt_cost(left, right, maxD - minD + 1, minD, cap), block)
+    ok, real = geom_valid(base, valid, D, minD, maxD)
+    k = np.clip(real - minD, 0, maxD - minD)
+    want = np.where(ok, np.take_along_axis(full, k, axis=2), MAX)
+    got = O.bt_window_cost_local(left, right, base, valid, D, maxD, block, minD, cap)
+    np.testing.assert_array_equal(got, want)
+
+
+# -------------------------- pyramid recurrences: real-disparity textbook
+def textbook_pyramid(C, base, valid, P1, P2, minD, maxD, direction, lut=None, img=None):
+    """Pyramid path costs along one direction ('lr', 'rl' or 'tb') in
+    real-disparity space.  A label exists iff its geometry is valid and its
+    cost is below MAX; every transition from the previous pixel's labels is
+    priced by |real - real'| (0, P1, P2'); a pixel whose previous pixel had no
+    path starts over at L = C.  P2' = lut[|I(p) - I(prev)|] when given."""
+    h, w, D = C.shape
+    ok, real = geom_valid(base, valid, D, minD, maxD)
+    ok &= C < MAX
+    out = np.full((h, w, D), MAX, np.int64)
+    if direction == "tb":
+        lines = [[(y, x) for y in range(h)] for x in range(w)]
+    elif direction == "lr":
+        lines = [[(y, x) for x in range(w)] for y in range(h)]
+    else:
+        lines = [[(y, x) for x in range(w - 1, -1, -1)] for y in range(h)]
+    for line in lines:
+        prev = None          # {real: L} of the previous pixel's labels with a path
+        prev_p = None
+        for (y, x) in line:
+            L = np.full(D, MAX, np.int64)
+            if not ok[y, x].any():
+                out[y, x] = L
+                prev, prev_p = None, (y, x)
+                continue
+            if prev is None:
+                L[ok[y, x]] = C[y, x][ok[y, x]]
+            else:
+                p2 = P2 if lut is None else int(lut[abs(int(img[y, x]) - int(img[prev_p]))])
+                mprev = min(prev.values())
+                for d in range(D):
+                    if not ok[y, x, d]:
+                        continue
+                    best = min(mprev + p2, MAX)
+                    for rp, lv in prev.items():
+                        dr = abs(rp - int(real[y, x, d]))
+                        best = min(best, lv + (0 if dr == 0 else P1 if dr == 1 else p2))
+                    L[d] = MAX if best >= MAX else min(MAX, int(C[y, x, d]) + best - mprev)
+            out[y, x] = L
+            cur = {int(real[y, x, d]): int(L[d]) for d in range(D) if L[d] < MAX}
+            prev = cur if cur else None
+            prev_p = (y, x)
+    return out
+
+
+def pyramid_case(seed, h=6, w=44, D=16, maxD=34):
+    r = rng(seed)
+    C = r.integers(0, 25, (h, w, D)).astype(np.int16)
+    base = np.clip(r.integers(-3, 4, (h, w)).cumsum(axis=1) + 5, 0, 22).astype(np.int16)
+    valid = (r.random((h, w)) > 0.1).astype(np.uint8)
+    C[r.random((h, w, D)) < 0.05] = MAX
+    img = r.integers(0, 256, (h, w), dtype=np.uint8)
+    return C, base, valid, img
+
+
+@pytest.mark.parametrize("adaptive", [False, True])
+def test_pyramid_tb_matches_textbook_and_accumulates(adaptive):
+    """sgm_kernels.cu:1263-1334: the T->B recurrence over columns, added to
+    the L->R result with the pyramid saturation: MAX if either is MAX, else
+    min(MAX - 1, old + L)."""
+    C, base, valid, img = pyramid_case(51 + adaptive)
+    maxD = 34
+    lut = O.p2_lut(12, 1, 2.0, 10.0, 0.6, 1.8)[0] if adaptive else None
+    acc0 = rng(52).integers(0, 30000, C.shape).astype(np.int16)
+    acc0[rng(53).random(C.shape) < 0.05] = MAX
+    got = O.path_pyr("tb", C, base, valid, img, 3, 12, maxD, acc=acc0, lut=lut)
+    L = textbook_pyramid(C, base, valid, 3, 12, 0, maxD, "tb", lut, img)
+    want = np.where((acc0 < MAX) & (L < MAX), np.minimum(MAX - 1, acc0.astype(np.int64) + L), MAX)
+    np.testing.assert_array_equal(got, want)
+
+
+def test_pyramid_lr_matches_textbook_adaptive():
+    C, base, valid, img = pyramid_case(54)
+    lut = O.p2_lut(12, 1, 2.0, 10.0, 0.6, 1.8)[0]
+    got = O.path_pyr("lr", C, base, valid, img, 3, 12, 34, lut=lut)
+    np.testing.assert_array_equal(got, textbook_pyramid(C, base, valid, 3, 12, 0, 34, "lr", lut, img))
+
+
+def pyramid_wta_numpy(S, lv, base, valid, D, minD, maxD, uniq, min_conf):
+    """Per-pixel selection of the pyramid R->L kernel (sgm_kernels.cu:1412-1478)
+    restated: argmin with ties to the smallest label; second best over existing
+    labels more than one label away; valid only with >= 3 geometric labels,
+    finite best and alternative, uniqueness, no artificial window edge (best at
+    label 0 while the window could start lower, or at label D-1 while it could
+    reach higher), and confidence >= min_conf; 1/16-px parabola when both
+    neighbours are finite; right-view projection keeps the lowest cost, and
+    among equal costs the pixel visited first (largest x)."""
+    h, w = S.shape[:2]
+    disp = np.full((h, w), (minD - 1) * 16, np.int16)
+    conf = np.zeros((h, w), np.float32)
+    d2 = np.full((h, w), (minD - 1) * 16, np.int16)
+    d2c = np.full((h, w), MAX, np.int64)
+    geo, _ = geom_valid(base, valid, D, minD, maxD)
+    for y in range(h):
+        for x in range(w - 1, -1, -1):
+            s = S[y, x]
+            vc = int(geo[y, x].sum())
+            b = int(s.argmin())
+            m = int(s[b])
+            alts = [int(s[d]) for d in range(D) if lv[y, x, d] and abs(d - b) > 1]
+            alt = min(alts) if alts else MAX
+            okp = vc >= 3 and m < MAX and alt < MAX
+            if okp and uniq > 0 and alt * (100 - uniq) < m * 100:
+                okp = False
+            bs = int(base[y, x])
+            if (b == 0 and bs > minD) or (b == D - 1 and bs + D - 1 < min(maxD, x)):
+                okp = False
+            cf = np.float32(0)
+            if okp:
+                cf = np.float32(np.float32(alt) - np.float32(m)) / np.float32(max(m, 1))
+                cf = np.float32(min(max(cf, np.float32(0)), np.float32(1)))
+                if cf < np.float32(min_conf):
+                    okp = False
+            if not okp:
+                continue
+            v = (bs + b) * 16
+            if 0 < b < D - 1 and max(s[b - 1], s[b], s[b + 1]) < MAX:
+                c0, c1, c2 = int(s[b - 1]), int(s[b]), int(s[b + 1])
+                den = max(c0 + c2 - 2 * c1, 1)
+                num = (c0 - c2) * 16 + den
+                v += abs(num) // (2 * den) * (1 if num >= 0 else -1)
+            disp[y, x], conf[y, x] = v, cf
+            x2 = x - (bs + b)
+            if 0 <= x2 < w and d2c[y, x2] > m:
+                d2c[y, x2], d2[y, x2] = m, bs + b
+    return disp, conf, d2, d2c.astype(np.int16)
+
+
+@pytest.mark.parametrize("uniq,min_conf,accumulate", [(10, 0.05, True), (0, 0.0, True),
+                                                      (25, 0.2, True), (10, 0.05, False)])
+def test_pyramid_rl_wta_brute_force(uniq, min_conf, accumulate):
+    """sgm_kernels.cu:1336-1478 on small volumes: the R->L recurrence from the
+    real-disparity textbook, the pyramid sum rule, then every selection rule
+    by brute force."""
+    C, base, valid, img = pyramid_case(60 + uniq, h=7, w=48)
+    D, maxD = 16, 34
+    r = rng(61)
+    acc0 = r.integers(0, 400, C.shape).astype(np.int16) if accumulate else None
+    if accumulate:
+        acc0[r.random(C.shape) < 0.03] = MAX
+    out = O.path_rl_pyr_wta(C, base, valid, img, 3, 12, maxD, uniq=uniq, min_conf=min_conf,
+                            acc=acc0)
+    L = textbook_pyramid(C, base, valid, 3, 12, 0, maxD, "rl")
+    geo, _ = geom_valid(base, valid, D, 0, maxD)
+    lv = geo & (C < MAX)
+    old = acc0.astype(np.int64) if accumulate else np.zeros(C.shape, np.int64)
+    S = np.where(lv & (L < MAX) & (old < MAX), np.minimum(MAX - 1, old + L), MAX)
+    np.testing.assert_array_equal(out.acc, S)
+    disp, conf, d2, d2c = pyramid_wta_numpy(S, lv, base, valid, D, 0, maxD, uniq, min_conf)
+    np.testing.assert_array_equal(out.disp, disp)
+    np.testing.assert_array_equal(out.conf, conf)
+    np.testing.assert_array_equal(out.disp2, d2)
+    np.testing.assert_array_equal(out.disp2_cost, d2c)
+    assert (disp != -16).sum() > 20          # the case exercises accepted pixels
+
+
+def test_pyramid_rl_wta_artificial_edges():
+    """A window whose cheapest label is its first (or last) label while the
+    window could extend that way is rejected; at the true range ends it is
+    kept (sgm_kernels.cu:1434-1440)."""
+    h, w, D, maxD = 1, 60, 16, 40
+    base = np.full((h, w), 10, np.int16)
+    valid = np.ones((h, w), np.uint8)
+    C = np.full((h, w, D), 20, np.int16)
+    C[..., 0] = 0                                  # best at label 0 everywhere
+    img = np.zeros((h, w), np.uint8)
+    out = O.path_rl_pyr_wta(C, base, valid, img, 0, 0, maxD, uniq=0, min_conf=0.0)
+    assert (out.disp[0, 25:] == -16).all()        # base 10 > minD 0: artificial left edge
+    base0 = np.zeros((h, w), np.int16)
+    out = O.path_rl_pyr_wta(C, base0, valid, img, 0, 0, maxD, uniq=0, min_conf=0.0)
+    assert (out.disp[0, 25:] == 0).all()          # base = minD: a true edge
+    C2 = np.full((h, w, D), 20, np.int16)
+    C2[..., D - 1] = 0                             # best at the last label
+    out = O.path_rl_pyr_wta(C2, base, valid, img, 0, 0, maxD, uniq=0, min_conf=0.0)
+    assert (out.disp[0, 41:] == -16).all()        # 25 < min(maxD, x): could reach higher
+    base_top = np.full((h, w), maxD - D + 1, np.int16)
+    out = O.path_rl_pyr_wta(C2, base_top, valid, img, 0, 0, maxD, uniq=0, min_conf=0.0)
+    assert (out.disp[0, 41:] == maxD * 16).all()  # window ends at maxD: a true edge
+
+
+# ---------------------------- single-scale R->L with penalties: textbook
+@pytest.mark.parametrize("P1,P2,adaptive,accumulate", [(3, 11, False, False), (7, 40, True, True),
+                                                       (4, 4, False, True)])
+def test_single_scale_rl_matches_mirrored_textbook(P1, P2, adaptive, accumulate):
+    """sgm_kernels.cu:665-735: the R->L path is the textbook recurrence on the
+    mirrored row (start L = C at x = W-1), added to the accumulator with
+    saturation at SHRT_MAX; WTA then follows the summed costs."""
+    r = rng(70 + P2)
+    h, w, D = 6, 52, 16
+    C = r.integers(0, 60, (h, w, D)).astype(np.int16)
+    img = r.integers(0, 256, (h, w), dtype=np.uint8)
+    lut = O.p2_lut(P2, 1, 2.0, 10.0, 0.6, 1.8)[0] if adaptive else None
+    acc0 = r.integers(0, 500, (h, w, D)).astype(np.int16) if accumulate else None
+    out = O.path_rl_wta(C, img, P1, P2, uniq=10, acc=acc0, lut=lut)
+    Lm = textbook_path(np.ascontiguousarray(C[:, ::-1]), P1, P2, lut,
+                       np.ascontiguousarray(img[:, ::-1]))[:, ::-1]
+    S = np.minimum(MAX, (acc0.astype(np.int64) if accumulate else 0) + Lm)
+    np.testing.assert_array_equal(out.acc, S)
+    # The selection on top of S, for pixels inside [D, W-1).
+    for y in range(h):
+        for x in range(D, w - 1):
+            s = S[y, x]
+            b, m = int(s.argmin()), int(s.min())
+            alt = min(int(s[d]) for d in range(D) if abs(d - b) > 1)
+            if alt * 90 < m * 100:
+                assert out.disp[y, x] == -16
+                continue
+            want = b * 16
+            if 0 < b < D - 1:
+                c0, c1, c2 = (int(v) for v in s[b - 1:b + 2])
+                den = max(c0 + c2 - 2 * c1, 1)
+                num = (c0 - c2) * 16 + den
+                want += abs(num) // (2 * den) * (1 if num >= 0 else -1)
+            assert out.disp[y, x] == want, (y, x)

@@ -118,6 +118,48 @@ def make_pair(width: int, height: int, max_disp: int, seed: int, regions: int = 
 
 
 # Workloads named by what they are (sizes, range, cost, pyramid depth).
+UNEVEN_SIZES = ((640, 480), (960, 540), (1280, 720), (1920, 1080))
+SERVING_SIZES = ((640, 480, 64, 0), (1280, 720, 128, 1), (1920, 1080, 256, 2),
+                 (2560, 1440, 256, 2), (3840, 2160, 512, 3))  # (w, h, D, pyramid_levels)
+
+
+def uneven_spec(i: int, seed0: int = 2000) -> dict:
+    """Pair i of the uneven batch: size from UNEVEN_SIZES, D from {64, 128,
+    256}, pyramid_levels 0-3 (one to four levels in all) and BT or Census,
+    each drawn uniformly from the pair's own seed; the 3x3 median is on for
+    the even-numbered half.  Its images come from make_pair with the same
+    seed and disparities up to D - 1."""
+    rng = np.random.default_rng(seed0 + i)
+    w, h = UNEVEN_SIZES[int(rng.integers(len(UNEVEN_SIZES)))]
+    D = (64, 128, 256)[int(rng.integers(3))]
+    levels = int(rng.integers(4))
+    cost = ("bt", "census")[int(rng.integers(2))]
+    return {"w": w, "h": h, "D": D, "levels": levels, "cost": cost, "median": i % 2 == 0,
+            "seed": seed0 + i}
+
+
+def serving_spec(j: int, seed0: int = 3000) -> dict:
+    """Request j of the serving stream: a size from SERVING_SIZES (with its D
+    and pyramid depth) and BT or Census, drawn from the request's own seed."""
+    rng = np.random.default_rng(seed0 + j)
+    w, h, D, levels = SERVING_SIZES[int(rng.integers(len(SERVING_SIZES)))]
+    cost = ("bt", "census")[int(rng.integers(2))]
+    return {"w": w, "h": h, "D": D, "levels": levels, "cost": cost, "median": True,
+            "seed": seed0 + j}
+
+
+def spec_pair(spec: dict) -> Pair:
+    return make_pair(spec["w"], spec["h"], spec["D"] - 1, seed=spec["seed"])
+
+
+def spec_pairs(specs, workers: int = 8) -> list:
+    """make_pair for many specs on a thread pool (numpy releases the GIL in
+    the heavy array operations)."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=max(1, workers)) as ex:
+        return list(ex.map(spec_pair, specs))
+
+
 def config_pair(name: str, index: int = 0) -> Pair:
     if name == "640x480_d64_bt_single":
         return make_pair(640, 480, 63, seed=0, regions=4)

@@ -164,3 +164,69 @@ def test_rejections():
     with pytest.raises(ValueError, match="unknown ticket"):
         s.wait(12345)
     s.close()
+
+
+def test_wait_ends_when_another_call_takes_the_ticket():
+    """wait() on a ticket that poll() retrieves first raises instead of
+    blocking forever; exactly one of the two callers gets the result."""
+    import threading
+    p = synth.make_pair(160, 96, 63, seed=404)
+    params = params_of(64, 1, 1, 1, 1)
+    srv = P.Server(max_batch=4)
+    try:
+        for rep in range(6):
+            L, R = dev(p.left), dev(p.right)
+            o = out(96, 160, torch.int16, 1234)
+            t = srv.submit(L, R, params, disp16=o)
+            box = {}
+
+            def waiter():
+                try:
+                    box["wait"] = srv.wait(t)
+                except ValueError as e:
+                    box["err"] = str(e)
+
+            th = threading.Thread(target=waiter)
+            th.start()
+            polled = None
+            deadline = time.time() + 60
+            while th.is_alive() and polled is None and time.time() < deadline:
+                polled = srv.poll(t)
+            th.join(timeout=60)
+            assert not th.is_alive(), "wait() still blocked after the ticket was retrieved"
+            got = ("wait" in box) + (polled is not None)
+            assert got == 1, (rep, box, polled)
+            if polled is not None:
+                assert "retrieved by another call" in box["err"]
+            torch.cuda.synchronize()
+    finally:
+        srv.close()
+
+
+def test_python_api_rejects_bad_tensors():
+    """Shape, dtype, device and row-stride checks before any launch."""
+    w, h = 96, 64
+    params = P.validate_params(P.Params(num_disparities=32, pyramid_levels=1,
+                                        cost_mode=P.CostMode.Census, p1=8, p2=32))
+    m = P.StereoMatcher(w, h, params)
+    L = torch.zeros((h, w), dtype=torch.uint8, device="cuda")
+    ok16 = torch.zeros((h, w), dtype=torch.int16, device="cuda")
+    bad = [
+        dict(left=torch.zeros((h - 1, w), dtype=torch.uint8, device="cuda")),   # too small
+        dict(disp16=torch.zeros((h, w), dtype=torch.float32, device="cuda")),   # wrong dtype
+        dict(disp16=torch.zeros((h, w), dtype=torch.int16)),                    # host tensor
+        dict(disp16=torch.zeros((w, h), dtype=torch.int16, device="cuda").t()), # column-major
+        dict(conf=torch.zeros((h, 2 * w), dtype=torch.float32, device="cuda")[:, ::2]),
+    ]
+    for kw in bad:
+        args = dict(left=L, right=L, disp16=ok16, disp32=None, conf=None)
+        args.update(kw)
+        with pytest.raises(ValueError):
+            m.compute(args["left"], args["right"], args["disp16"], args["disp32"], args["conf"])
+    m.compute(L, L, ok16)          # a correct call still runs
+    torch.cuda.synchronize()
+    m.close()
+    srv = P.Server(max_batch=2)
+    with pytest.raises(ValueError):
+        srv.submit(L, L, params, disp16=torch.zeros((h, w), dtype=torch.float32, device="cuda"))
+    srv.close()

@@ -82,6 +82,50 @@ int pyrsgm_compute_single_scale(
     hipStream_t stream);
 
 /*
+ * Size of the fixed device scratch array behind
+ * pyrsgm_compute_single_scale_noalloc: 192 MiB, held by every device the
+ * function runs on until the process ends.  It fits a workspace of
+ * W*H*(8*Dp + 30) bytes (Dp = numDisparities rounded up to 16 * 2^k), e.g.
+ * 640x480 with 64 disparities.
+ */
+#define PYRSGM_NOALLOC_SCRATCH_BYTES (192u << 20)
+
+/*
+ * pyrsgm_compute_single_scale without a workspace argument and without any
+ * device allocation: the same kernels run on a fixed device array of
+ * PYRSGM_NOALLOC_SCRATCH_BYTES per device.  No counterpart in the reference;
+ * it is for callers that cannot allocate (work recorded under stream
+ * capture, serving loops and harnesses that forbid hipMalloc).  Arguments,
+ * outputs and error behaviour are those of pyrsgm_compute_single_scale;
+ * returns -1 (with a message on stderr) when the workspace for
+ * (width, height, numDisparities) exceeds the array.  All calls on one device
+ * share the array: calls that overlap in time on one device race, so
+ * serialise them (one stream, or events between streams).
+ */
+int pyrsgm_compute_single_scale_noalloc(
+    const unsigned char* d_left_gray,
+    const unsigned char* d_right_gray,
+    short* d_disp,
+    float* d_disp_float,
+    float* d_conf_float,
+    size_t left_step_bytes,
+    size_t right_step_bytes,
+    size_t disp_step_bytes,
+    size_t disp_float_step_bytes,
+    size_t conf_float_step_bytes,
+    int width, int height,
+    int numDisparities, int minDisparity,
+    int blockSize, int P1, int P2,
+    int uniquenessRatio, int disp12MaxDiff,
+    int preFilterCap,
+    int costMode,
+    int postFlags,
+    int numPaths,
+    int adaptiveP2,
+    float p2Alpha, float p2Beta, float p2LowerMult, float p2UpperMult,
+    hipStream_t stream);
+
+/*
  * One level of the coarse-to-fine pyramid.  Replaces reference
  * pyrsgm_c.h:99-128.
  *   refineLevel == 0: full search over [minDisparity, minDisparity+numDisparities)

@@ -154,6 +154,7 @@ def lib() -> C.CDLL:
         L.pyrsgm_destroy_workspace.restype = None
         L.pyrsgm_workspace_reserve.argtypes = [vp, ci, ci, ci]
         L.pyrsgm_compute_single_scale.argtypes = [vp] * 6 + [sz] * 5 + [ci] * 14 + [cf] * 4 + [vp]
+        L.pyrsgm_compute_single_scale_noalloc.argtypes = [vp] * 5 + [sz] * 5 + [ci] * 14 + [cf] * 4 + [vp]
         L.pyrsgm_compute_level.argtypes = [vp] * 8 + [sz] * 7 + [ci] * 17 + [cf, ci] + [cf] * 4 + [vp]
         L.pyrsgm_pyrdown_u8.argtypes = [vp, sz, ci, ci, vp, sz, ci, ci, vp]
         L.pyrsgm_validate_params.argtypes = [C.POINTER(_ParamsC), C.POINTER(_ParamsC), C.c_char_p, ci]
@@ -248,6 +249,27 @@ def compute_single_scale(workspace, d_left_gray, d_right_gray, d_disp, d_disp_fl
         conf_float_step_bytes, width, height, numDisparities, minDisparity, blockSize, P1, P2,
         uniquenessRatio, disp12MaxDiff, preFilterCap, costMode, postFlags, numPaths, adaptiveP2,
         p2Alpha, p2Beta, p2LowerMult, p2UpperMult, stream)
+
+
+def compute_single_scale_noalloc(d_left_gray, d_right_gray, d_disp, d_disp_float, d_conf_float,
+                                 left_step_bytes, right_step_bytes, disp_step_bytes,
+                                 disp_float_step_bytes, conf_float_step_bytes, width, height,
+                                 numDisparities, minDisparity, blockSize, P1, P2,
+                                 uniquenessRatio, disp12MaxDiff, preFilterCap, costMode,
+                                 postFlags, numPaths, adaptiveP2, p2Alpha, p2Beta, p2LowerMult,
+                                 p2UpperMult, stream=0) -> int:
+    """pyrsgm_compute_single_scale on the library's fixed device scratch array
+    (no allocation; NOALLOC_SCRATCH_BYTES per device; overlapping calls on one
+    device race)."""
+    return lib().pyrsgm_compute_single_scale_noalloc(
+        d_left_gray, d_right_gray, d_disp, d_disp_float, d_conf_float, left_step_bytes,
+        right_step_bytes, disp_step_bytes, disp_float_step_bytes, conf_float_step_bytes, width,
+        height, numDisparities, minDisparity, blockSize, P1, P2, uniquenessRatio, disp12MaxDiff,
+        preFilterCap, costMode, postFlags, numPaths, adaptiveP2, p2Alpha, p2Beta, p2LowerMult,
+        p2UpperMult, stream)
+
+
+NOALLOC_SCRATCH_BYTES = 192 << 20  # PYRSGM_NOALLOC_SCRATCH_BYTES
 
 
 def compute_level(workspace, d_left_gray, d_right_gray, d_parent_disp, d_parent_conf, d_disp,

@@ -30,6 +30,7 @@ SOURCES = [
     "kernels_cost.hip",
     "kernels_post.hip",
     "kernels_io.hip",
+    "noalloc.hip",
     "capi.cpp",
     "stereo_matcher.cpp",
     "matcher_capi.cpp",

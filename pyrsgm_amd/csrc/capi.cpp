@@ -54,22 +54,14 @@ size_t workspaceBytes(int W, int H, int D) { return layoutFor(W, H, D).total; }
 
 void releaseWorkspace(Workspace* ws)
 {
-    if (ws->base) (void)hipFree(ws->base);
+    if (ws->base && !ws->borrowed) (void)hipFree(ws->base);
     *ws = Workspace{};
 }
 
-int ensureWorkspace(Workspace* ws, int W, int H, int D, int n)
+namespace {
+// Points ws's buffers into n slices of layout l starting at base.
+void carve(Workspace* ws, void* base, const Layout& l, int n, int W, int H, int D)
 {
-    if (!ws || n < 1) return -1;
-    // A workspace holding at least n slices of this geometry serves n pairs.
-    if (ws->base && ws->W == W && ws->H == H && ws->D == D && ws->n >= n) return 0;
-    releaseWorkspace(ws);
-    const Layout l = layoutFor(W, H, D);
-    void* base = nullptr;
-    if (hipMalloc(&base, l.total * (size_t)n) != hipSuccess) {
-        (void)hipGetLastError();
-        return -1;
-    }
     char* b = static_cast<char*>(base);
     ws->base = base;
     ws->bytes = l.total * (size_t)n;
@@ -88,6 +80,34 @@ int ensureWorkspace(Workspace* ws, int W, int H, int D, int n)
     ws->disp = reinterpret_cast<int16_t*>(b + l.disp);
     ws->conf = reinterpret_cast<float*>(b + l.conf);
     ws->keys = reinterpret_cast<uint32_t*>(b + l.keys);
+}
+} // namespace
+
+int ensureWorkspace(Workspace* ws, int W, int H, int D, int n)
+{
+    if (!ws || n < 1) return -1;
+    // A workspace holding at least n slices of this geometry serves n pairs.
+    if (ws->base && ws->W == W && ws->H == H && ws->D == D && ws->n >= n) return 0;
+    if (ws->borrowed) return -1;
+    releaseWorkspace(ws);
+    const Layout l = layoutFor(W, H, D);
+    void* base = nullptr;
+    if (hipMalloc(&base, l.total * (size_t)n) != hipSuccess) {
+        (void)hipGetLastError();
+        return -1;
+    }
+    carve(ws, base, l, n, W, H, D);
+    return 0;
+}
+
+int borrowWorkspace(Workspace* ws, void* base, size_t bytes, int W, int H, int D)
+{
+    if (!ws || !base || (reinterpret_cast<uintptr_t>(base) % kAlign) != 0) return -1;
+    const Layout l = layoutFor(W, H, D);
+    if (l.total > bytes) return -1;
+    *ws = Workspace{};
+    carve(ws, base, l, 1, W, H, D);
+    ws->borrowed = true;
     return 0;
 }
 

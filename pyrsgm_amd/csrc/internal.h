@@ -58,11 +58,15 @@ struct Workspace {
     int16_t* disp = nullptr;    // [P] raw disparity*16 after WTA
     float* conf = nullptr;      // [P] raw confidence after WTA
     uint32_t* keys = nullptr;   // [P] right-view projection keys
+    bool borrowed = false;      // a view of caller-owned memory: never freed or reallocated
 };
 
 size_t workspaceBytes(int W, int H, int D);
+// A one-slice workspace view of `bytes` bytes at `base` (256-byte aligned
+// device memory that the caller owns) for (W, H, D); -1 if it does not fit.
+int borrowWorkspace(Workspace* ws, void* base, size_t bytes, int W, int H, int D);
 // (Re)allocates n slices of one level's buffers unless the geometry matches
-// and at least n slices are there.
+// and at least n slices are there (a borrowed view is never reallocated: -1).
 int ensureWorkspace(Workspace* ws, int W, int H, int D, int n = 1);
 void releaseWorkspace(Workspace* ws);
 

@@ -19,13 +19,16 @@ LIBDIR = os.environ.get("PYRSGM_LIBDIR", os.path.join(HERE, "lib"))
 OBJDIR = os.path.join(LIBDIR, "obj")
 LIB = os.path.join(LIBDIR, "libpyrsgm.so")
 
-SOURCES = [
-    "kernels_path_refine.hip",
-    "kernels_path_coarse.hip",
-    "kernels_path_single.hip",
-    "kernels_pathq_refine.hip",
-    "kernels_pathq_coarse.hip",
-    "kernels_pathq_single.hip",
+# kernels_path.hip once per (mode, adaptive P2, volume form, per-launch |
+# queued): about 170 kernel instantiations in 32 units that compile in
+# parallel.  Single scale has no exact form.
+PATH_PARTS = [
+    ("kernels_path.hip", f"kernels_path_m{m}a{a}f{f}{'q' if q else ''}",
+     {"PYRSGM_PATH_MODE": m, "PYRSGM_PATH_ADAPT": a, "PYRSGM_PATH_FORM": f, "PYRSGM_PATH_QUEUED": q})
+    for m in (2, 1, 0) for a in (0, 1) for f in (0, 1, 2) for q in (0, 1) if not (m == 0 and f == 0)
+]
+
+SOURCES = PATH_PARTS + [
     "kernels_wta.hip",
     "kernels_cost.hip",
     "kernels_post.hip",
@@ -70,13 +73,19 @@ def _deps_mtime() -> float:
     return max(os.path.getmtime(h) for h in hdrs)
 
 
-def _compile(src: str, force: bool = True) -> str:
-    obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
+def _compile(entry, force: bool = True) -> str:
+    """entry: a source file name, or (source, object stem, {macro: value})."""
+    if isinstance(entry, str):
+        src, stem, defines = entry, os.path.splitext(entry)[0], {}
+    else:
+        src, stem, defines = entry
+    obj = os.path.join(OBJDIR, stem + ".o")
     if not force and os.path.exists(obj):
         newest = max(os.path.getmtime(os.path.join(CSRC, src)), _deps_mtime())
         if os.path.getmtime(obj) > newest:
             return obj
-    cmd = [hipcc(), *common_flags(), "-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [hipcc(), *common_flags(), *[f"-D{k}={v}" for k, v in defines.items()],
+           "-c", os.path.join(CSRC, src), "-o", obj]
     if src.endswith(".cpp"):
         cmd.insert(1, "-x")
         cmd.insert(2, "hip")

@@ -168,11 +168,18 @@ int narrowVolumes(const LevelCall& c)
     return 24 + 2 * p2max < 255 ? 1 : 0;
 }
 
+PathPart& pathPart(int mode, int adaptive, int form)
+{
+    // Filled by the kernels_path.hip units' static initialisers (construct on
+    // first use, so their order does not matter).
+    static PathPart table[3][2][3];
+    return table[mode][adaptive ? 1 : 0][form];
+}
+
 hipError_t launchPaths(const PathArgs& a, int mode, int adaptive, hipStream_t s)
 {
-    if (mode == kRefine) return launchPathsRefine(a, adaptive, s);
-    if (mode == kCoarse) return launchPathsCoarse(a, adaptive, s);
-    return launchPathsSingle(a, adaptive, s);
+    const PathPart& p = pathPart(mode, adaptive, pathForm(mode, a.exact, a.narrow));
+    return p.launch ? p.launch(a, s) : hipErrorInvalidValue;
 }
 
 int queuedLinesPerBlock(int Dp, int paired)
@@ -183,20 +190,34 @@ int queuedLinesPerBlock(int Dp, int paired)
 
 int pairedLinesAuto(size_t blocks4) { return blocks4 >= kPairedWaves ? 1 : 0; }
 
+namespace {
+const void* queuedKernelOf(int mode, int Dp, int adaptive, int exact, int narrow, int paired)
+{
+    const PathPart& p = pathPart(mode, adaptive, pathForm(mode, exact, narrow));
+    return p.queued ? p.queued(Dp, paired) : nullptr;
+}
+} // namespace
+
 int queuedBlocksPerCU(int mode, int Dp, int adaptive, int exact, int narrow, int paired)
 {
-    if (mode == kRefine) return queuedOccupancyRefine(Dp, adaptive, exact, narrow, paired);
-    if (mode == kCoarse) return queuedOccupancyCoarse(Dp, adaptive, exact, narrow, paired);
-    return queuedOccupancySingle(Dp, adaptive, exact, narrow, paired);
+    const void* k = queuedKernelOf(mode, Dp, adaptive, exact, narrow, paired);
+    int n = 0;
+    if (!k || hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, 128, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        return 1;
+    }
+    return n > 0 ? n : 1;
 }
 
 hipError_t launchPathsQueued(const QueueArgs& q, int mode, int Dp, int adaptive, int exact,
                              int narrow, int paired, int grid, hipStream_t s)
 {
     if (q.total <= 0) return hipSuccess;
-    if (mode == kRefine) return launchPathsQueuedRefine(q, Dp, adaptive, exact, narrow, paired, grid, s);
-    if (mode == kCoarse) return launchPathsQueuedCoarse(q, Dp, adaptive, exact, narrow, paired, grid, s);
-    return launchPathsQueuedSingle(q, Dp, adaptive, narrow, grid, s);
+    const void* k = queuedKernelOf(mode, Dp, adaptive, exact, narrow, paired);
+    if (!k || grid < 1) return hipErrorInvalidValue;
+    QueueArgs args = q;
+    void* params[] = {&args};
+    return hipLaunchKernel(k, dim3(grid), dim3(128), params, 0, s);
 }
 
 hipError_t launchLevel(const LevelCall& c, Workspace& ws, hipStream_t s)

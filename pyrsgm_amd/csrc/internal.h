@@ -149,9 +149,21 @@ struct PathArgs {
 };
 
 hipError_t launchPaths(const PathArgs& a, int mode, int adaptive, hipStream_t s);
-hipError_t launchPathsSingle(const PathArgs& a, int adaptive, hipStream_t s);
-hipError_t launchPathsCoarse(const PathArgs& a, int adaptive, hipStream_t s);
-hipError_t launchPathsRefine(const PathArgs& a, int adaptive, hipStream_t s);
+
+// Path kernels of one (mode, adaptive P2, volume form) specialisation; each is
+// compiled in a unit of its own (kernels_path.hip) and registers itself here
+// when the library loads.  Form: exact pyramid step (path costs may reach
+// SHRT_MAX), byte volumes, or int16 volumes.
+constexpr int kFormExact = 0, kFormNarrow = 1, kFormWide = 2;
+struct PathPart {
+    hipError_t (*launch)(const PathArgs& a, hipStream_t s) = nullptr;  // per-launch kPaths
+    const void* (*queued)(int Dp, int paired) = nullptr;               // kPathsQ kernel
+};
+PathPart& pathPart(int mode, int adaptive, int form);
+inline int pathForm(int mode, int exact, int narrow)
+{
+    return (mode != kSingle && exact) ? kFormExact : (narrow ? kFormNarrow : kFormWide);
+}
 hipError_t launchWTA(const PathArgs& a, int mode, hipStream_t s);
 
 // Parts of a level a call enqueues: the descriptor pass (census words / BT
@@ -197,17 +209,8 @@ int pairedLinesAuto(size_t blocks4);
 constexpr size_t kPairedWaves = 4096;
 // Workgroups of one queued launch that are resident at once per CU (occupancy).
 int queuedBlocksPerCU(int mode, int Dp, int adaptive, int exact, int narrow, int paired);
-int queuedOccupancySingle(int Dp, int adaptive, int exact, int narrow, int paired);
-int queuedOccupancyCoarse(int Dp, int adaptive, int exact, int narrow, int paired);
-int queuedOccupancyRefine(int Dp, int adaptive, int exact, int narrow, int paired);
 hipError_t launchPathsQueued(const QueueArgs& q, int mode, int Dp, int adaptive, int exact,
                              int narrow, int paired, int grid, hipStream_t s);
-hipError_t launchPathsQueuedSingle(const QueueArgs& q, int Dp, int adaptive, int narrow, int grid,
-                                   hipStream_t s);
-hipError_t launchPathsQueuedCoarse(const QueueArgs& q, int Dp, int adaptive, int exact,
-                                   int narrow, int paired, int grid, hipStream_t s);
-hipError_t launchPathsQueuedRefine(const QueueArgs& q, int Dp, int adaptive, int exact,
-                                   int narrow, int paired, int grid, hipStream_t s);
 // Winner-take-all of many pairs' levels of one specialisation (mode, Dp,
 // byte volumes) in one launch: block b belongs to task t with
 // start[t] <= b < start[t + 1], task t having wtaBlocks(tasks[t]) blocks.

@@ -1146,9 +1146,19 @@ hipError_t dispatchPaths(const PathArgs& a, hipStream_t s)
     case 64: kPaths<64, 1, MODE, ADAPT, EXACT, false><<<grid, 128, 0, s>>>(b); break;
     case 128: kPaths<64, 2, MODE, ADAPT, EXACT, false><<<grid, 128, 0, s>>>(b); break;
     case 256: kPaths<64, 4, MODE, ADAPT, EXACT, false><<<grid, 128, 0, s>>>(b); break;
-    case 512: kPaths<64, 8, MODE, ADAPT, EXACT, false><<<grid, 128, 0, s>>>(b); break;
-    case 1024: kPaths<64, 16, MODE, ADAPT, EXACT, false><<<grid, 128, 0, s>>>(b); break;
-    default: return hipErrorInvalidValue;
+    default:
+        // Refinement levels hold at most 256 labels (the entry point rejects more).
+        if constexpr (MODE != kRefine) {
+            if (a.Dp == 512) {
+                kPaths<64, 8, MODE, ADAPT, EXACT, false><<<grid, 128, 0, s>>>(b);
+                break;
+            }
+            if (a.Dp == 1024) {
+                kPaths<64, 16, MODE, ADAPT, EXACT, false><<<grid, 128, 0, s>>>(b);
+                break;
+            }
+        }
+        return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
@@ -1174,59 +1184,13 @@ const void* queuedKernel(int Dp, int paired)
     case 64: return reinterpret_cast<const void*>(&kPathsQ<64, 1, MODE, ADAPT, EXACT, false>);
     case 128: return reinterpret_cast<const void*>(&kPathsQ<64, 2, MODE, ADAPT, EXACT, false>);
     case 256: return reinterpret_cast<const void*>(&kPathsQ<64, 4, MODE, ADAPT, EXACT, false>);
-    case 512: return reinterpret_cast<const void*>(&kPathsQ<64, 8, MODE, ADAPT, EXACT, false>);
-    case 1024: return reinterpret_cast<const void*>(&kPathsQ<64, 16, MODE, ADAPT, EXACT, false>);
-    default: return nullptr;
+    default:
+        if constexpr (MODE != kRefine) {
+            if (Dp == 512) return reinterpret_cast<const void*>(&kPathsQ<64, 8, MODE, ADAPT, EXACT, false>);
+            if (Dp == 1024) return reinterpret_cast<const void*>(&kPathsQ<64, 16, MODE, ADAPT, EXACT, false>);
+        }
+        return nullptr;
     }
-}
-
-template <int MODE>
-const void* queuedKernelMode(int Dp, int adaptive, int exact, int narrow, int paired)
-{
-    if (MODE != kSingle && exact)
-        return adaptive ? queuedKernel<MODE, true, true, false>(Dp, 0)
-                        : queuedKernel<MODE, false, true, false>(Dp, 0);
-    if (narrow)
-        return adaptive ? queuedKernel<MODE, true, false, true>(Dp, paired)
-                        : queuedKernel<MODE, false, false, true>(Dp, paired);
-    return adaptive ? queuedKernel<MODE, true, false, false>(Dp, 0)
-                    : queuedKernel<MODE, false, false, false>(Dp, 0);
-}
-
-template <int MODE>
-int queuedOccupancy(int Dp, int adaptive, int exact, int narrow, int paired)
-{
-    const void* k = queuedKernelMode<MODE>(Dp, adaptive, exact, narrow, paired);
-    int n = 0;
-    if (!k || hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, 128, 0) != hipSuccess) {
-        (void)hipGetLastError();
-        return 1;
-    }
-    return n > 0 ? n : 1;
-}
-
-template <int MODE>
-hipError_t launchQueuedMode(const QueueArgs& q, int Dp, int adaptive, int exact, int narrow,
-                            int paired, int grid, hipStream_t s)
-{
-    const void* k = queuedKernelMode<MODE>(Dp, adaptive, exact, narrow, paired);
-    if (!k || grid < 1) return hipErrorInvalidValue;
-    QueueArgs args = q;
-    void* params[] = {&args};
-    return hipLaunchKernel(k, dim3(grid), dim3(128), params, 0, s);
-}
-
-template <int MODE>
-hipError_t launchPathsMode(const PathArgs& a, int adaptive, hipStream_t s)
-{
-    if (MODE != kSingle && a.exact)
-        return adaptive ? dispatchPaths<MODE, true, true, false>(a, s)
-                        : dispatchPaths<MODE, false, true, false>(a, s);
-    if (a.narrow)
-        return adaptive ? dispatchPaths<MODE, true, false, true>(a, s)
-                        : dispatchPaths<MODE, false, false, true>(a, s);
-    return adaptive ? dispatchPaths<MODE, true, false, false>(a, s)
-                    : dispatchPaths<MODE, false, false, false>(a, s);
 }
 
 } // namespace path

@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import concurrent.futures as cf
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -99,12 +100,51 @@ def _compile(entry, force: bool = True) -> str:
     return obj
 
 
+# On gfx950 a reversed-operand VOP2 instruction with a DPP lane swizzle
+# (v_subrev_u32_dpp and the like) applies the swizzle to its second source,
+# while the compiler's DPP combiner assumes the first (measured: v_subrev_u32_dpp
+# with row_shr:1 returns y[l-1] - x[l], not y[l] - x[l-1]).  The combiner forms
+# such instructions when it folds a DPP move whose out-of-row lanes read 0 into
+# a subtraction or shift, so every build is checked for them.
+_REV_DPP = re.compile(r"^\s+(v_[a-z0-9_]*rev[a-z0-9_]*_dpp)\s", re.M)
+
+
+def _llvm_tool(name: str) -> str:
+    for cand in (os.path.join(os.path.dirname(os.path.realpath(hipcc())), "..", "llvm", "bin", name),
+                 f"/opt/rocm/llvm/bin/{name}", shutil.which(name)):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError(f"{name} not found")
+
+
+def check_no_reversed_dpp(obj: str) -> None:
+    """Raises if the gfx950 code of `obj` holds a reversed-operand DPP instruction."""
+    import tempfile
+    with tempfile.TemporaryDirectory() as td:
+        fat, dev = os.path.join(td, "fat.bin"), os.path.join(td, "dev.co")
+        r = subprocess.run(["objcopy", f"--dump-section=.hip_fatbin={fat}", obj, os.path.join(td, "o")],
+                           capture_output=True, text=True)
+        if r.returncode != 0 or not os.path.exists(fat):
+            return  # no device code in this object
+        subprocess.run([_llvm_tool("clang-offload-bundler"), "--unbundle", "--type=o", f"--input={fat}",
+                        f"--targets=hipv4-amdgcn-amd-amdhsa--{ARCH}", f"--output={dev}"],
+                       check=True, capture_output=True)
+        dis = subprocess.run([_llvm_tool("llvm-objdump"), "-d", f"--mcpu={ARCH}", dev],
+                             check=True, capture_output=True, text=True).stdout
+    bad = sorted(set(_REV_DPP.findall(dis)))
+    if bad:
+        raise RuntimeError(f"{os.path.basename(obj)}: reversed-operand DPP instructions {bad} "
+                           "(their swizzle applies to the second source on gfx950); keep the DPP "
+                           "move out of the combiner (see dppOpaque in device_common.h)")
+
+
 def build(jobs: int | None = None, force: bool = True) -> str:
     """Compile every source (force) or only out-of-date ones, then link."""
     os.makedirs(OBJDIR, exist_ok=True)
     jobs = jobs or min(len(SOURCES), max(1, (os.cpu_count() or 4)))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(lambda s: _compile(s, force), SOURCES))
+        list(ex.map(check_no_reversed_dpp, objs))
     tmp = LIB + ".tmp"
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -268,6 +268,7 @@ hipError_t launchLevel(const LevelCall& c, Workspace& ws, hipStream_t s)
     static const int remap = getenv("PYRSGM_NO_XCD_REMAP") ? 0 : 1;
     a.xcdRemap = c.tiles.xcdRemap == 0 ? remap : (c.tiles.xcdRemap == 2 ? 1 : 0);
     a.pairedMode = c.tiles.paired;
+    a.modularMode = c.tiles.modular;
     a.disp = ws.disp;
     a.conf = ws.conf;
     a.keys = ws.keys;

@@ -26,6 +26,8 @@ enum : int {
     kDppWaveShl1 = 0x130,      // lane i <- lane i+1 across the wave
     kDppWaveShr1 = 0x138,      // lane i <- lane i-1 across the wave
     kDppRowMirror = 0x140,     // lane i <- lane 15-i inside its row
+    kDppRowRor1 = 0x121,       // lane i <- lane (i-1) mod 16 inside its row
+    kDppRowRor15 = 0x12F,      // lane i <- lane (i+1) mod 16 inside its row
     kDppRowHalfMirror = 0x141, // lane i <- lane 7-i inside its half row
 };
 
@@ -34,6 +36,20 @@ template <int CTRL>
 __device__ __forceinline__ int dpp(int old, int src)
 {
     return __builtin_amdgcn_update_dpp(old, src, CTRL, 0xF, 0xF, false);
+}
+
+// The same move kept out of the compiler's DPP combiner.  On gfx950 a
+// reversed-operand VOP2 instruction with DPP (v_subrev_u32_dpp, v_lshlrev_*_dpp,
+// ...) applies the lane swizzle to its second source, while the combiner, which
+// folds a move whose out-of-row lanes read 0 into its user, assumes the first;
+// x - dpp(y) could become such an instruction.  The build rejects any object
+// that contains one.
+template <int CTRL>
+__device__ __forceinline__ int dppOpaque(int old, int src)
+{
+    int v = dpp<CTRL>(old, src);
+    asm volatile("" : "+v"(v));
+    return v;
 }
 
 __device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }

@@ -76,6 +76,7 @@ struct TileConfig {
     int localRows = 0;  // rows per block of the refinement census cost: 0 (= 4), 2, 4 or 8
     int paired = 0;     // paired lines for 16-label byte volumes: 0 by launch size, 1 off, 2 on
     int xcdRemap = 0;   // XCD-contiguous T->B column groups: 0 default, 1 off, 2 on
+    int modular = 0;    // modular refinement step: 0 by launch size, 1 off, 2 on
 };
 
 // Everything one level needs; filled by the C-ABI entry points.
@@ -137,6 +138,7 @@ struct PathArgs {
     int narrow;                // byte volumes (see narrowVolumes)
     int xcdRemap;              // XCD-contiguous line groups (on unless PYRSGM_NO_XCD_REMAP is set)
     int pairedMode;            // TileConfig::paired
+    int modularMode;           // TileConfig::modular
     int nLRBlocks;             // combined launch: blocks [0, nLR) run L->R,
     int nRLBlocks;             // [nLR, nLR+nRL) R->L, the rest T->B
     int16_t* disp;
@@ -207,6 +209,9 @@ int pairedLinesAuto(size_t blocks4);
 // Path launches of at least this many 4-line waves use paired lines (byte
 // volumes, 16 labels, pyramid levels): four per SIMD.
 constexpr size_t kPairedWaves = 4096;
+// Per-launch 16-label byte-volume refinement path launches of at most this many
+// workgroups use the modular step (measured: faster at 510, slower at 1020).
+constexpr size_t kModularMaxBlocks = 768;
 // Workgroups of one queued launch that are resident at once per CU (occupancy).
 int queuedBlocksPerCU(int mode, int Dp, int adaptive, int exact, int narrow, int paired);
 hipError_t launchPathsQueued(const QueueArgs& q, int mode, int Dp, int adaptive, int exact,

@@ -226,8 +226,8 @@ __device__ __forceinline__ int gmax(int v)
 // lastPair gives that value and the other half's (a permlane swap), so that
 // gmax(v) = max(a, b); the path step folds that level into its subtraction:
 // x - gmax(v) = min(x - a, x - b).  A 16-lane line keeps the whole reduction
-// in gmaxPartial (a = b): folding its last DPP level (row_mirror) into the
-// subtraction's v_subrev_u32_dpp gave wrong results on gfx950.
+// in gmaxPartial (a = b): x - row_mirror(a) would be a v_subrev_u32_dpp,
+// whose swizzle gfx950 applies to x instead of a (see dppOpaque).
 template <int G>
 __device__ __forceinline__ int gmaxPartial(int v)
 {
@@ -349,7 +349,7 @@ struct ChunkAddr {
 //    step); vertical path [step][line][label].  A horizontal line's row and a
 //    vertical step's row are each one contiguous run of the path-cost volume,
 //    written with 16-byte stores.
-template <int G, int K, bool NARROW = false, bool PK = false>
+template <int G, int K, bool NARROW = false, bool PK = false, bool MODR = false>
 struct Lds {
     static_assert(!NARROW || K == 1, "byte volumes have one label per lane");
     static_assert(!PK || (NARROW && G == 16), "paired lines need byte volumes of 16 labels");
@@ -385,7 +385,13 @@ struct Lds {
     static constexpr int NQ = PIECES / 64;
     static constexpr int OFF_W = 3 * CSLOT;
     static constexpr int OFF_T = OFF_W + 3 * WSLOT;
-    static constexpr int BYTES = OFF_T + 2 * TSLOT;
+    // Modular refinement step (MODR): per ring slot, three lane-mask slots laid
+    // out like a cost slot (a byte per compute lane and step), and per chunk
+    // (4 in flight) the rotation of each of the memory wave's pieces.
+    static constexpr int OFF_M = OFF_T + 2 * TSLOT;
+    static constexpr int MSLOT = MODR ? 3 * CSLOT : 0;
+    static constexpr int OFF_R = OFF_M + 3 * MSLOT;
+    static constexpr int BYTES = OFF_R + (MODR ? 4 * 64 : 0);
 };
 
 // Workgroup barrier ordering LDS only: the memory wave's global loads stay in
@@ -407,8 +413,62 @@ __device__ __forceinline__ void waveLdsFence()
     asm volatile("" ::: "memory");
 }
 
+// The modular refinement step (see runLine): 16-label byte-volume refinement
+// lines in the fast form, when the launch asks for it (MOD).
+template <int G, int K, int MODE, bool EXACT, bool NARROW, bool PK, bool MOD>
+constexpr bool modularRefine()
+{
+    return MOD && MODE == kRefine && !EXACT && NARROW && !PK && G == 16 && K == 1;
+}
+
+// Byte i of a 16-byte vector goes to byte (i - R) & 15: rotation toward lower
+// bytes by R (0..15).
+__device__ __forceinline__ u4v rotDown16(const u4v& x, int R)
+{
+    const bool q1 = (R & 4) != 0, q2 = (R & 8) != 0;
+    u4v y, z;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) y[i] = q1 ? x[(i + 1) & 3] : x[i];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) z[i] = q2 ? y[(i + 2) & 3] : y[i];
+    u4v o;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) o[i] = __builtin_amdgcn_alignbyte(z[(i + 1) & 3], z[i], (uint32_t)(R & 3));
+    return o;
+}
+
+// 16 mask bits -> 16 bytes of 0x00 / 0xFF (bit i -> byte i).
+__device__ __forceinline__ u4v bitsToBytes16(uint32_t m)
+{
+    u4v o;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) o[i] = (((m >> (4 * i)) & 15u) * 0x00204081u & 0x01010101u) * 0xFFu;
+    return o;
+}
+
+// x & (sign-extended byte SEL of m): m's bytes are 0x00 or 0xFF.
+template <int SEL>
+__device__ __forceinline__ int andByteMask(int x, uint32_t m)
+{
+    int r;
+    if constexpr (SEL == 0)
+        asm("v_and_b32_sdwa %0, %1, sext(%2) dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_0"
+            : "=v"(r) : "v"(x), "v"(m));
+    else if constexpr (SEL == 1)
+        asm("v_and_b32_sdwa %0, %1, sext(%2) dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_1"
+            : "=v"(r) : "v"(x), "v"(m));
+    else if constexpr (SEL == 2)
+        asm("v_and_b32_sdwa %0, %1, sext(%2) dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_2"
+            : "=v"(r) : "v"(x), "v"(m));
+    else
+        asm("v_and_b32_sdwa %0, %1, sext(%2) dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_3"
+            : "=v"(r) : "v"(x), "v"(m));
+    return r;
+}
+
 // MODE: kSingle, kCoarse or kRefine.
-template <int G, int K, int MODE, bool ADAPT, bool EXACT, bool NARROW, bool PK, int DIR>
+template <int G, int K, int MODE, bool ADAPT, bool EXACT, bool NARROW, bool PK, int DIR,
+          bool MOD = false>
 __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int lineGroup,
                                         const int* s_lut, int* s_xchg, char* s_lds)
 {
@@ -422,7 +482,10 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
     constexpr bool PYR = MODE != kSingle;
     constexpr bool REFINE = MODE == kRefine;
     constexpr bool WORDS = REFINE || ADAPT;  // per-step words are loaded
-    using TL = Lds<G, K, NARROW, PK>;
+    // Modular refinement step: compute lane g holds the label whose real
+    // disparity is congruent to g modulo 16 (see the step below).
+    constexpr bool MODR = modularRefine<G, K, MODE, EXACT, NARROW, PK, MOD>();
+    using TL = Lds<G, K, NARROW, PK, MODR>;
     constexpr int ESZ = TL::ESZ;
     constexpr int VB = GK * ESZ;  // bytes of one pixel's label vector
     constexpr int W32 = TL::W32;
@@ -529,26 +592,74 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
         }
     };
     // Scatter a loaded chunk into cost slot k % 3: label d of step u goes to
-    // compute lane (line * G + d / K), dword u * W32 + (d % K) / 2.
+    // compute lane (line * G + d / K), dword u * W32 + (d % K) / 2.  MODR: the
+    // label-order costs of a pixel are rotated into lane order first, and the
+    // step's three lane masks go to the mask slots with the same layout.
+    int mcarry = 0;  // MODR: window base of each line's last step so far (all lanes of its row)
     auto mstore = [&](const MBuf& m, int k) {
         const int sc = chunkStart(k);
         const int nSteps = imin(U, s1 - sc);
         const int nRows = VERT ? nSteps : nValid;
         const int nCols = VERT ? nValid : nSteps;
         char* const slot = s_lds + (k % 3) * TL::CSLOT;
+        int info = 0;  // MODR: lane (line, r): window shift + 32 (bits 0-7), base & 15 (bits 8-11)
+        if constexpr (MODR) {
+            const int base = (int)(int16_t)(m.w[0] & 0xFFFF);
+            const int prevBase = dppOpaque<kDppRowShr1>(mcarry, base);
+            mcarry = rowBcast<U - 1>(base);
+            info = (imin(imax(base - prevBase, -17), 17) + 32) | ((base & 15) << 8);
+        }
 #pragma unroll
         for (int i = 0; i < TL::NQ; ++i) {
             const int pc = lane + 64 * i;
             const int row = pc / PPR;
             const int ob = (pc - row * PPR) * 16;
             const int col = ob / VB;
+            int pinfo = 0;
+            if constexpr (MODR) {
+                // (shift, rotation) of this piece's (line, step), from the lane
+                // that loaded that step's word (every lane takes part).
+                const int jj = VERT ? col : row;
+                const int u = VERT ? row : (RL ? nSteps - 1 - col : col);
+                const int src = imin(imax(jj, 0), LPW - 1) * 16 + imin(imax(u, 0), 15);
+                pinfo = __builtin_amdgcn_ds_bpermute(src << 2, info);
+                *reinterpret_cast<uint8_t*>(s_lds + TL::OFF_R + (k & 3) * 64 + pc) = (uint8_t)(pinfo >> 8);
+            }
             if (row < nRows && col < nCols) {
                 const int jj = VERT ? col : row;
                 const int u = VERT ? row : (RL ? nSteps - 1 - col : col);
                 const int d0 = (ob - col * VB) / ESZ;
                 // Paired lines: line jj goes to half jj / LPW of lane (jj % LPW, label).
                 char* const p = slot + TL::laneOff((jj % LPW) * G + d0 / K) + (jj / LPW) * 16;
-                const u4v q = m.q[i];
+                u4v q = m.q[i];
+                if constexpr (MODR) {
+                    // Lane g takes label (g - rot) & 15, the label whose real
+                    // disparity (base + label) is congruent to g modulo 16.
+                    const int sh = (pinfo & 0xFF) - 32, rot = (pinfo >> 8) & 15;
+                    q = rotDown16(q, (16 - rot) & 15);
+                    // Masks in label order o: the previous pixel's path cost for
+                    // real base + o + e (e = 0, -1, +1) exists iff o + sh + e lies in
+                    // [0, 15]; then rotated to lane order and widened to bytes.
+                    auto range = [](int lo, int hi) {
+                        const int a = imin(imax(lo, 0), 16), b = imin(imax(hi + 1, 0), 16);
+                        return b > a ? ((1u << b) - 1u) & ~((1u << a) - 1u) : 0u;
+                    };
+                    auto toLanes = [&](uint32_t v) {
+                        return bitsToBytes16(((v | (v << 16)) >> (16 - rot)) & 0xFFFFu);
+                    };
+                    const u4v m0 = toLanes(range(-sh, 15 - sh));
+                    const u4v mm = toLanes(range(1 - sh, 16 - sh));
+                    const u4v mp = toLanes(range(-1 - sh, 14 - sh));
+                    char* const pm = s_lds + TL::OFF_M + (k % 3) * TL::MSLOT +
+                                     TL::laneOff((jj % LPW) * G) + u;
+#pragma unroll
+                    for (int t = 0; t < 16; ++t) {
+                        const int sft = 8 * (t & 3);
+                        *reinterpret_cast<uint8_t*>(pm + t * TL::LANE) = (uint8_t)(m0[t >> 2] >> sft);
+                        *reinterpret_cast<uint8_t*>(pm + TL::CSLOT + t * TL::LANE) = (uint8_t)(mm[t >> 2] >> sft);
+                        *reinterpret_cast<uint8_t*>(pm + 2 * TL::CSLOT + t * TL::LANE) = (uint8_t)(mp[t >> 2] >> sft);
+                    }
+                }
                 if constexpr (NARROW) {
 #pragma unroll
                     for (int t = 0; t < 16; ++t)
@@ -594,7 +705,12 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
             const int row = pc / PPR;
             const int ob = (pc - row * PPR) * 16;
             const int tbyte = row * (VERT ? LPWC * VB : TL::ROWH) + ob;
-            const u4v v = *reinterpret_cast<const u4v*>(tile + tbyte);
+            u4v v = *reinterpret_cast<const u4v*>(tile + tbyte);
+            if constexpr (MODR) {
+                // Lane order back to label order.
+                const int rot = *reinterpret_cast<const uint8_t*>(s_lds + TL::OFF_R + (k & 3) * 64 + pc);
+                v = rotDown16(v, rot);
+            }
             const bool ok = row < nRows && ob / VB < nCols;
             const uint32_t go = ok ? (uint32_t)row * volRow + (uint32_t)ob : kOffLimit;
             __builtin_amdgcn_raw_buffer_store_b128(v, rOut, go, 0, 0);
@@ -685,9 +801,18 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
 
     // Pixel step u of a chunk.  cw: complemented costs; p2: this step's P2'
     // (line-uniform); shv: per-step window shifts of this lane's line (slot u is
-    // read on moved steps only); bigMask bit u: some line shifts by more than 1.
+    // read on moved steps only); bigMask bit u: some line shifts by more than 1;
+    // mk: MODR lane masks of the chunk (byte u of dword u / 4).
+    //
+    // MODR: lane g holds the label whose real disparity r is congruent to g
+    // modulo 16, so a window move changes no register: the previous pixel's
+    // path cost at real r is this lane's own value and those at r - 1, r + 1
+    // are the row rotations by one lane, each valid iff that real was in the
+    // previous pixel's window (masks m0, mm, mp, precomputed per step by the
+    // memory wave).  Invalid operands read 0 ("no path"), which is what
+    // sgm_kernels.cu:1237-1243 gives them (SHRT_MAX before the complement).
     auto step = [&](auto uc, const uint32_t (&cw)[(K + 1) / 2], int p2, bool moved,
-                    uint32_t bigMask, int shv, char* tp) {
+                    uint32_t bigMask, int shv, char* tp, const u4v* mk) {
         constexpr int u = decltype(uc)::value;
         int e[K];
 #pragma unroll
@@ -697,14 +822,26 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
             // Neighbours of an unmoved window; a moved one overrides them (one
             // uniform branch per step, taken on the rare moved steps).
             int h0[K], y[K];
-            const int lowN = lowerOr0<G>(Hs[K - 1], g);
-            const int upN = upperOr0<G>(Hs[0], g);
+            int lowN = 0, upN = 0;
+            if constexpr (MODR) {
+                (void)moved;
+                (void)bigMask;
+                (void)shv;
+                constexpr int B = u & 3, Q = u >> 2;
+                lowN = andByteMask<B>(__builtin_amdgcn_mov_dpp(Hs[0], kDppRowRor1, 0xF, 0xF, false), mk[1][Q]);
+                upN = andByteMask<B>(__builtin_amdgcn_mov_dpp(Hs[0], kDppRowRor15, 0xF, 0xF, false), mk[2][Q]);
+                h0[0] = andByteMask<B>(Hs[0], mk[0][Q]);
+                y[0] = imax(lowN, upN);
+            } else {
+                lowN = lowerOr0<G>(Hs[K - 1], g);
+                upN = upperOr0<G>(Hs[0], g);
 #pragma unroll
-            for (int k = 0; k < K; ++k) {
-                h0[k] = Hs[k];
-                y[k] = imax(k > 0 ? Hs[k - 1] : lowN, k < K - 1 ? Hs[k + 1] : upN);
+                for (int k = 0; k < K; ++k) {
+                    h0[k] = Hs[k];
+                    y[k] = imax(k > 0 ? Hs[k - 1] : lowN, k < K - 1 ? Hs[k + 1] : upN);
+                }
             }
-            if (REFINE && __builtin_expect(moved, 0)) {
+            if (!MODR && REFINE && __builtin_expect(moved, 0)) {
                 if (G == 16 && K == 1 && !((bigMask >> u) & 1u)) {
                     // Every line of the wave moved by -1, 0 or +1 labels: the
                     // re-indexed values are row shifts of the previous ones (zero
@@ -784,12 +921,18 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
         u4v q[TL::CQ];
         int w;
         int w2;  // paired lines: the second line's word
+        u4v mk[MODR ? 3 : 1];  // MODR lane masks (m0, mm, mp)
     };
     // This lane's costs of chunk k (and its row's per-step words) from LDS.
     auto cread = [&](CBuf& c, int k) {
         const char* const p = s_lds + (k % 3) * TL::CSLOT + TL::laneOff(lane);
 #pragma unroll
         for (int i = 0; i < TL::CQ; ++i) c.q[i] = *reinterpret_cast<const u4v*>(p + 16 * i);
+        if constexpr (MODR) {
+            const char* const pm = s_lds + TL::OFF_M + (k % 3) * TL::MSLOT + TL::laneOff(lane);
+#pragma unroll
+            for (int i = 0; i < 3; ++i) c.mk[i] = *reinterpret_cast<const u4v*>(pm + i * TL::CSLOT);
+        }
         if constexpr (WORDS) {
             const char* const wbase = s_lds + TL::OFF_W + (k % 3) * TL::WSLOT;
             c.w = *reinterpret_cast<const int*>(wbase + (j * 16 + r16) * 4);
@@ -930,7 +1073,7 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
         const int sc = chunkStart(k);
         int p2v = 0, shv = 0;
         uint32_t moveMask = 0, bigMask = 0;
-        if constexpr (REFINE) {
+        if constexpr (REFINE && !MODR) {
             const int base = (int)(int16_t)(c.w & 0xFFFF);
             const int prevBase = dpp<kDppRowShr1>(carryBase, base);
             shv = base - prevBase;
@@ -974,7 +1117,7 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
                     tb = (u * LPW + j) * VB + g * K * ESZ;
                 else
                     tb = j * TL::ROWH + (RL ? (full ? U - 1 - u : nSteps - 1 - u) : u) * VB + g * K * ESZ;
-                step(uc, cw, p2, moved, bigMask, shv, tile + tb);
+                step(uc, cw, p2, moved, bigMask, shv, tile + tb, c.mk);
             }
         };
         unrollSteps<U>(run);
@@ -1024,12 +1167,15 @@ __device__ __forceinline__ int xcdRemap(int b, int n)
     return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
 }
 
-template <int G, int K, int MODE, bool ADAPT, bool EXACT, bool NARROW, bool PK = false>
+// MOD: 16-label byte-volume refinement lines use the modular step.
+template <int G, int K, int MODE, bool ADAPT, bool EXACT, bool NARROW, bool PK = false,
+          bool MOD = false>
 __global__ __launch_bounds__(128) void kPaths(PathArgs a)
 {
     __shared__ int s_lut[256];
     __shared__ int s_xchg[MODE == kRefine ? 64 * K : 1];
-    __shared__ __attribute__((aligned(16))) char s_lds[Lds<G, K, NARROW, PK>::BYTES];
+    __shared__ __attribute__((aligned(16))) char
+        s_lds[Lds<G, K, NARROW, PK, modularRefine<G, K, MODE, EXACT, NARROW, PK, MOD>()>::BYTES];
     if constexpr (ADAPT) {
         for (int i = threadIdx.x; i < 256; i += 128) s_lut[i] = a.lut[i];
         __syncthreads();
@@ -1037,13 +1183,14 @@ __global__ __launch_bounds__(128) void kPaths(PathArgs a)
     const int b = blockIdx.x;
     const size_t pr = blockIdx.y;  // pair
     if (b < a.nLRBlocks)
-        runLine<G, K, MODE, ADAPT, EXACT, NARROW, PK, kLR>(a, pr, b, s_lut, s_xchg, s_lds);
+        runLine<G, K, MODE, ADAPT, EXACT, NARROW, PK, kLR, MOD>(a, pr, b, s_lut, s_xchg, s_lds);
     else if (b < a.nLRBlocks + a.nRLBlocks)
-        runLine<G, K, MODE, ADAPT, EXACT, NARROW, PK, kRL>(a, pr, b - a.nLRBlocks, s_lut, s_xchg, s_lds);
+        runLine<G, K, MODE, ADAPT, EXACT, NARROW, PK, kRL, MOD>(a, pr, b - a.nLRBlocks, s_lut, s_xchg,
+                                                                s_lds);
     else {
         const int t = b - a.nLRBlocks - a.nRLBlocks, nTB = (int)gridDim.x - a.nLRBlocks - a.nRLBlocks;
-        runLine<G, K, MODE, ADAPT, EXACT, NARROW, PK, kTB>(a, pr, a.xcdRemap ? xcdRemap(t, nTB) : t,
-                                                           s_lut, s_xchg, s_lds);
+        runLine<G, K, MODE, ADAPT, EXACT, NARROW, PK, kTB, MOD>(
+            a, pr, a.xcdRemap ? xcdRemap(t, nTB) : t, s_lut, s_xchg, s_lds);
     }
 }
 
@@ -1128,6 +1275,15 @@ hipError_t dispatchPaths(const PathArgs& a, hipStream_t s)
             const int nTB8 = a.numPaths >= 3 ? (a.W + 7) / 8 : 0;
             const dim3 grid8(c.nLRBlocks + c.nRLBlocks + nTB8, a.batch.n);
             kPaths<16, 1, MODE, ADAPT, EXACT, true, true><<<grid8, 128, 0, s>>>(c);
+            return hipGetLastError();
+        }
+    }
+    if constexpr (NARROW && MODE == kRefine && !EXACT) {
+        // The modular step has no per-step branch but a heavier data-moving
+        // wave; it is faster up to about 3 recurrence waves per 4 SIMDs.
+        if (a.Dp == 16 && (a.modularMode == 2 ||
+                           (a.modularMode == 0 && (size_t)blocks * a.batch.n <= kModularMaxBlocks))) {
+            kPaths<16, 1, MODE, ADAPT, EXACT, true, false, true><<<grid, 128, 0, s>>>(b);
             return hipGetLastError();
         }
     }

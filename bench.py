@@ -205,15 +205,6 @@ def bench_batched(P, torch, log, n=32, W=960, H=540, D=128):
     return out
 
 
-def _mixed_requests(rng, n, kinds, pool):
-    """n requests (key, pair) drawn from kinds; pair images from the per-size pool."""
-    reqs = []
-    for j in range(n):
-        k = kinds[int(rng.integers(len(kinds)))]
-        reqs.append((k, pool[(k["w"], k["h"])][j % len(pool[(k["w"], k["h"])])]))
-    return reqs
-
-
 def _params_for(P, k):
     census = k["cost"] == "census"
     return P.Params(num_disparities=k["D"], pyramid_levels=k["levels"],
@@ -246,32 +237,9 @@ def _sequential_service_ms(P, torch, reqs, dev_pairs, outs):
     return svc
 
 
-def _fifo(arrivals_ms, service_ms):
-    """Latencies of a single FIFO server with the given service times."""
-    t, lat = 0.0, []
-    for a, s in zip(arrivals_ms, service_ms):
-        t = max(t, a) + s
-        lat.append(t - a)
-    return lat
-
-
 def _pct(v, p):
     v = sorted(v)
     return v[min(len(v) - 1, max(0, int(-(-p * len(v) // 1)) - 1))] if v else None
-
-
-def _pool(P, sizes, per_size, seed0, log):
-    from pyrsgm_amd import synth
-    import torch
-    pool, dev_pairs = {}, {}
-    for (w, h, dmax) in sizes:
-        pool[(w, h)] = []
-        for i in range(per_size):
-            p = synth.make_pair(w, h, dmax - 1, seed=seed0 + 10 * len(pool) + i)
-            pool[(w, h)].append(p)
-            dev_pairs[id(p)] = (torch.from_numpy(p.left).cuda(), torch.from_numpy(p.right).cuda())
-    log(f"[bench] pool: {sum(len(v) for v in pool.values())} pairs")
-    return pool, dev_pairs
 
 
 def _serve_burst(P, torch, server, reqs, dev_pairs, outs):
@@ -287,23 +255,26 @@ def _serve_burst(P, torch, server, reqs, dev_pairs, outs):
     return res, (t1 - t0) * 1e-9
 
 
+def _spec_requests(specs, log, what):
+    """(key, pair) per spec, each pair generated from its own seed with
+    disparities up to its D, and the pairs' device images."""
+    import torch
+    from pyrsgm_amd import synth
+    pairs = synth.spec_pairs(specs, workers=min(16, os.cpu_count() or 1))
+    dev_pairs = {id(p): (torch.from_numpy(p.left).cuda(), torch.from_numpy(p.right).cuda())
+                 for p in pairs}
+    reqs = [({k: v for k, v in s.items() if k != "seed"}, p) for s, p in zip(specs, pairs)]
+    log(f"[bench] {what}: {len(reqs)} pairs generated (one seed each)")
+    return reqs, dev_pairs
+
+
 def _uneven_requests(P, log, n=64, seed=2000):
-    """The uneven workload: n pairs drawn uniformly from {640x480, 960x540,
-    1280x720, 1920x1080} x D {64,128,256} x pyramid_levels {0..3} x {BT,
-    census} x median on/off."""
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    sizes = [(640, 480, 256), (960, 540, 256), (1280, 720, 256), (1920, 1080, 256)]
-    kinds = []
-    for (w, h, _) in sizes:
-        for D in (64, 128, 256):
-            for lv in (0, 1, 2, 3):
-                for cost in ("bt", "census"):
-                    for med in (True, False):
-                        kinds.append({"w": w, "h": h, "D": D, "levels": lv, "cost": cost,
-                                      "median": med})
-    pool, dev_pairs = _pool(P, sizes, 2, seed, log)
-    return _mixed_requests(rng, n, kinds, pool), dev_pairs
+    """The uneven workload: pair i has its own seed (seed + i), from which its
+    size ({640x480, 960x540, 1280x720, 1920x1080}), D ({64, 128, 256}),
+    pyramid_levels (0..3) and cost (BT / census) are drawn uniformly, and its
+    images, with disparities up to its D; the median is on for every other pair."""
+    from pyrsgm_amd import synth
+    return _spec_requests([synth.uneven_spec(i, seed) for i in range(n)], log, "uneven")
 
 
 def profile_pair(P, torch, spec, reps=10):
@@ -359,8 +330,9 @@ def bench_uneven(P, torch, log, n=64, seed=2000):
     server.close()
     seq_pps = n / (sum(svc) * 1e-3)
     mixed = bench_mixed(P, torch, log, reqs, dev_pairs, outs, ref, seq_pps)
-    out = {"workload": f"{n} pairs, sizes 640x480..1920x1080, D 64/128/256, levels 0-3, "
-                       "BT/census, median on/off", "pairs": n,
+    out = {"workload": f"{n} pairs, one seed each ({seed}+i): sizes 640x480..1920x1080, "
+                       "D 64/128/256, levels 0-3, BT/census, median on every other pair, "
+                       "disparities up to each pair's D", "pairs": n,
            "distinct_keys": len({tuple(sorted(k.items())) for k, _ in reqs}),
            "pairs_per_s": round(n / span, 1),
            "per_pair_pairs_per_s": round(seq_pps, 1),
@@ -381,6 +353,20 @@ def _oracle_frames(args):
     for _ in range(reps):
         O.matcher_compute(left, right, params, want32=False, wantc=False)
     return time.perf_counter() - t0
+
+
+def host_cpu():
+    """CPU model and logical-core count of this host (the CPU baseline's machine)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count()}
 
 
 def cpu_all_cores(left, right, params, workers, reps, log):
@@ -496,22 +482,39 @@ def bench_mixed(P, torch, log, reqs, dev_pairs, outs, ref, seq_pps):
     return out
 
 
-def bench_serving(P, torch, log, nreq=400, seed=3000):
+def _sequential_served(P, torch, reqs, dev_pairs, outs, arrivals_ms, matchers):
+    """The plain per-pair path served sequentially, executed: requests are taken
+    in arrival order, each waits for its arrival time if the server is idle, is
+    matched alone through its key's StereoMatcher and synchronised before the
+    next starts.  Returns per-request latencies (completion - arrival, ms)."""
+    import time
+    lat = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for j, (k, pair) in enumerate(reqs):
+        due = t0 + arrivals_ms[j] * 1e-3
+        while time.perf_counter() < due:
+            pass
+        Ld, Rd = dev_pairs[id(pair)]
+        matchers[tuple(sorted(k.items()))].compute(Ld, Rd, outs[j], None, None)
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - due) * 1e3)
+    return lat
+
+
+def bench_serving(P, torch, log, nreq=200, seed=3000):
     """Open-loop Poisson stream of pairs sized {640x480, 1280x720, 1920x1080,
     2560x1440, 3840x2160} (D 64/128/256/256/512; pyramid_levels 0/1/2/2/3;
-    BT or census per request) through the serving loop, at arrival rates swept
-    as fractions of its measured capacity.  The same stream served sequentially
-    through StereoMatcher is the comparison: its latencies come from each
-    request's measured device time in a FIFO queue with the same arrivals."""
+    BT or census per request); request j has its own seed (seed + j), from
+    which its size and cost are drawn and its images generated.  The stream goes
+    through the serving loop at arrival rates swept as fractions of the loop's
+    measured capacity; the comparison is the same stream served sequentially
+    through StereoMatcher, executed with the same arrival times."""
     import numpy as np
+    from pyrsgm_amd import synth
     rng = np.random.default_rng(seed)
-    sizes = [(640, 480, 64), (1280, 720, 128), (1920, 1080, 256), (2560, 1440, 256),
-             (3840, 2160, 512)]
-    levels = {640: 0, 1280: 1, 1920: 2, 2560: 2, 3840: 3}
-    kinds = [{"w": w, "h": h, "D": d, "levels": levels[w], "cost": c, "median": True}
-             for (w, h, d) in sizes for c in ("bt", "census")]
-    pool, dev_pairs = _pool(P, sizes, 2, seed, log)
-    reqs = _mixed_requests(rng, nreq, kinds, pool)
+    reqs, dev_pairs = _spec_requests([synth.serving_spec(j, seed) for j in range(nreq)], log,
+                                     "serving")
     outs = [torch.empty((k["h"], k["w"]), dtype=torch.int16, device="cuda") for k, _ in reqs]
     svc = _sequential_service_ms(P, torch, reqs, dev_pairs, outs)
     ref = [o.clone() for o in outs]
@@ -523,6 +526,11 @@ def bench_serving(P, torch, log, nreq=400, seed=3000):
     same = all(bool((a == b).all()) for a, b in zip(outs, ref))
     log(f"[bench] serving capacity {capacity:.1f} pairs/s (sequential {seq_capacity:.1f}); "
         f"same as alone: {same}")
+    matchers = {}
+    for k, _ in reqs:
+        key = tuple(sorted(k.items()))
+        if key not in matchers:
+            matchers[key] = P.StereoMatcher(k["w"], k["h"], _params_for(P, k))
     rates = []
     for frac in (0.5, 0.8, 0.95):
         lam = frac * capacity
@@ -541,7 +549,7 @@ def bench_serving(P, torch, log, nreq=400, seed=3000):
         res = [server.wait(t) for t in tickets]
         st = server.stats()
         lat = [r.latency_ms for r in res]
-        seq_lat = _fifo(list(arr * 1e3), svc)
+        seq_lat = _sequential_served(P, torch, reqs, dev_pairs, outs, list(arr * 1e3), matchers)
         rates.append({"offered_pairs_per_s": round(lam, 1),
                       "pairs_per_s": round(st.pairs_per_s, 1),
                       "p50_ms": round(_pct(lat, 0.5), 3), "p99_ms": round(_pct(lat, 0.99), 3),
@@ -549,13 +557,18 @@ def bench_serving(P, torch, log, nreq=400, seed=3000):
                       "sequential_p50_ms": round(_pct(seq_lat, 0.5), 3),
                       "sequential_p99_ms": round(_pct(seq_lat, 0.99), 3)})
         log(f"[bench] serving at {lam:.0f}/s: {rates[-1]}")
+    for m in matchers.values():
+        m.close()
     st = server.stats()
     server.close()
-    return {"workload": "Poisson stream, sizes 640x480/1280x720/1920x1080/2560x1440/3840x2160, "
-                        "D 64/128/256/256/512, levels 0/1/2/2/3, BT or census, disp16 out",
+    return {"workload": f"Poisson stream of {nreq} requests, one seed each ({seed}+j): sizes "
+                        "640x480/1280x720/1920x1080/2560x1440/3840x2160, D 64/128/256/256/512, "
+                        "levels 0/1/2/2/3, BT or census, disp16 out",
             "requests_per_rate": nreq, "capacity_pairs_per_s": round(capacity, 1),
             "sequential_capacity_pairs_per_s": round(seq_capacity, 1),
             "speedup_vs_sequential": round(capacity / seq_capacity, 3),
+            "sequential_baseline": "executed: StereoMatcher per request in arrival order, "
+                                   "synchronised, same arrival times",
             "bit_identical_to_alone": same, "rates": rates,
             "peak_device_bytes": st.peak_bytes, "lanes": st.lanes}
 
@@ -569,7 +582,8 @@ def main() -> int:
     ap.add_argument("--cpu-reps", type=int, default=12, help="oracle runs in the CPU baseline")
     ap.add_argument("--cpu-workers", type=int, default=min(16, os.cpu_count() or 1),
                     help="oracle worker processes of the all-cores CPU baseline (1: single thread "
-                         "only)")
+                         "only); the default is every core up to 16, the CPU share a one-GPU "
+                         "box grants a job")
     ap.add_argument("--no-kernel-events", action="store_true",
                     help="time the steps without per-kernel HIP events (no roofline)")
     ap.add_argument("--breakdown-steps", type=int, default=20,
@@ -589,7 +603,7 @@ def main() -> int:
                     help="skip the 640x480 D=64 BT single-scale leg (and its OpenCV probe)")
     ap.add_argument("--no-serve", action="store_true",
                     help="skip the uneven-batch and serving legs")
-    ap.add_argument("--serve-requests", type=int, default=400,
+    ap.add_argument("--serve-requests", type=int, default=200,
                     help="requests per arrival rate in the serving leg")
     ap.add_argument("--only-batch", action="store_true",
                     help="run only the 32-pair batched leg and print its JSON (for profiling)")
@@ -761,7 +775,10 @@ def main() -> int:
             ref = O.matcher_compute(pair.left, pair.right, params)
         c1 = time.perf_counter()
         single_fps = reps / (c1 - c0)
+        host = host_cpu()
+        label = "CPU fallback (not OpenCV): the C oracle"
         cpu = {"value": round(single_fps, 4), "unit": "frames/s", "cores": 1, "kind": "port",
+               "label": label, "cpu_model": host["model"], "nproc": host["nproc"], "threads": 1,
                "sample": f"{reps} runs of the full 1920x1080 D=256 L2-census pair (seed 1) through "
                          f"the C oracle, {c1 - c0:.2f} s total, single thread"}
         allc = None
@@ -775,7 +792,8 @@ def main() -> int:
                 log(f"[bench] cpu all cores failed: {type(e).__name__}: {e}")
         if allc:
             cpu = {"value": allc["fps"], "unit": "frames/s", "cores": allc["workers"],
-                   "kind": "port",
+                   "kind": "port", "label": label, "cpu_model": host["model"],
+                   "nproc": host["nproc"], "threads": allc["workers"],
                    "sample": f"{allc['frames']} runs of the full 1920x1080 D=256 L2-census pair "
                              f"(seed 1) through the C oracle in {allc['workers']} worker processes "
                              f"(one thread each), {allc['wall_s']:.2f} s wall",

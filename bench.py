@@ -120,6 +120,71 @@ def derive_traffic(bench_json, fetch_csv, write_csv, cal_fetch_csv, cal_write_cs
     return res
 
 
+def kernel_table(bench_json, trace_csv):
+    """Markdown table of the library's kernel durations from a rocprofv3 --kernel-trace
+    CSV.  A bench line with a per-frame launch sequence: launches matched to it in
+    dispatch order (kernel names checked against their stage), one row per kernel and
+    stage@level.  A MixedBatch line (with "passes"): one row per kernel, totals per pass.
+    Otherwise one row per kernel and grid."""
+    import csv
+    with open(bench_json) as f:
+        info = next(json.loads(l) for l in f if l.startswith("{"))
+    seq = info.get("launch_sequence")
+    rows = []
+    with open(trace_csv, newline="") as f:
+        for r in csv.DictReader(f):
+            if "pyr::" in r["Kernel_Name"]:
+                rows.append(r)
+    rows.sort(key=lambda r: int(r["Dispatch_Id"]))
+    groups = {}
+    for i, r in enumerate(rows):
+        name = r["Kernel_Name"]
+        short = name.replace("(anonymous namespace)::", "").replace("pyr::", "")
+        short = short.split("(")[0].replace("void ", "")
+        grid = "x".join(r.get(f"Grid_Size_{a}", "?") for a in "XYZ")
+        us = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1000.0
+        if seq:
+            tag = seq[i % len(seq)]
+            stage = tag.split("@")[0]
+            if not any(stem in name for stem in STAGE_KERNELS[stage]):
+                raise SystemExit(f"launch {i} is {name}, expected a {stage} kernel")
+            key = (short, tag, grid)
+        elif "passes" in info:
+            key = (short,)
+        else:
+            key = (short, grid)
+        groups.setdefault(key, []).append(us)
+    if seq:
+        head = "| kernel | stage@level (W x H x labels) | grid | launches | mean us | min us | max us |"
+    elif "passes" in info:
+        head = "| kernel | launches per pass | ms per pass | min us | max us |"
+    else:
+        head = "| kernel | grid | launches | mean us | min us | max us |"
+    out = [head, "|" + "---|" * (head.count("|") - 1)]
+    if "passes" in info and not seq:
+        n = info["passes"]
+        for (short,), v in sorted(groups.items(), key=lambda kv: -sum(kv[1])):
+            out.append(f"| `{short}` | {len(v) / n:g} | {sum(v) / n / 1000.0:.3f} | "
+                       f"{min(v):.1f} | {max(v):.1f} |")
+        out.append("")
+        out.append(f"Launches: {len(rows)} over {n} passes; kernel time per pass "
+                   f"{sum(sum(v) for v in groups.values()) / n / 1000.0:.2f} ms (launches on "
+                   f"different streams overlap, so this exceeds the pass time "
+                   f"{info.get('ms_per_pass', 0):.2f} ms).")
+        return "\n".join(out)
+    for key, v in sorted(groups.items(), key=lambda kv: -sum(kv[1]) / len(kv[1])):
+        mean = sum(v) / len(v)
+        out.append("| `" + key[0] + "` | " + " | ".join(key[1:]) +
+                   f" | {len(v)} | {mean:.2f} | {min(v):.2f} | {max(v):.2f} |")
+    if seq:
+        frames = len(rows) / len(seq)
+        total = sum(sum(v) for v in groups.values()) / frames
+        out.append("")
+        out.append(f"Launches: {len(rows)} ({frames:g} frames of {len(seq)}).  "
+                   f"Kernel time per frame: {total:.1f} us.")
+    return "\n".join(out)
+
+
 def _event_ms(torch, fn, reps, warm=3):
     """Mean device milliseconds of fn() over reps runs on the current stream."""
     for _ in range(warm):
@@ -597,6 +662,9 @@ def main() -> int:
                     help="turn rocprofv3 FETCH_SIZE / WRITE_SIZE counter CSVs of a bench run "
                          "(and of a --copy-calibration run) into per-launch HBM bytes, "
                          "printed as JSON")
+    ap.add_argument("--kernel-table", nargs=2, metavar=("BENCH_JSON", "TRACE_CSV"),
+                    help="turn a rocprofv3 --kernel-trace CSV of a bench run into a markdown "
+                         "table of per-launch durations by kernel and stage@level")
     ap.add_argument("--calib-mib", type=int, default=512)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched leg")
     ap.add_argument("--no-single", action="store_true",
@@ -619,6 +687,9 @@ def main() -> int:
     args = ap.parse_args()
     if args.derive_traffic:
         print(json.dumps(derive_traffic(*args.derive_traffic, args.calib_mib << 20), indent=1))
+        return 0
+    if args.kernel_table:
+        print(kernel_table(*args.kernel_table))
         return 0
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration << 20)

@@ -52,6 +52,33 @@ __device__ __forceinline__ int dppOpaque(int old, int src)
     return v;
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin in linear order (x
+// fastest, then y, then z), and each XCD has its own L2.  xcdRemap(b, n) is a
+// bijection of [0, n) under which the blocks of one XCD (equal b % 8) get a
+// contiguous range of indices.
+__device__ __forceinline__ int xcdRemap(int b, int n)
+{
+    const int q = n >> 3, r = n & 7, x = b & 7, i = b >> 3;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+}
+
+// The tile (x, y, z) of this workgroup when tiles are handed to the XCDs in
+// contiguous raster ranges: neighbouring tiles, which read parts of the same
+// 128-byte lines and share halo rows, then share one L2 instead of each XCD
+// fetching them.
+struct Tile3 {
+    int x, y, z;
+};
+__device__ __forceinline__ Tile3 xcdTile()
+{
+    const int nx = (int)gridDim.x, ny = (int)gridDim.y;
+    const int n = nx * ny * (int)gridDim.z;
+    const int b = (int)blockIdx.x + nx * ((int)blockIdx.y + ny * (int)blockIdx.z);
+    const int t = xcdRemap(b, n);
+    const int txy = t % (nx * ny);
+    return Tile3{txy % nx, txy / nx, t / (nx * ny)};
+}
+
 __device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
 __device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }
 __device__ __forceinline__ uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }

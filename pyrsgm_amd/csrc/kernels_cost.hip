@@ -78,13 +78,14 @@ __global__ __launch_bounds__(256) void kPrepCensus(const uint8_t* left, int lste
 {
     constexpr int TW = 68, TS = 72, TH = kCenRows + 4, TILE = TW * TH, NL = (TILE + 255) / 256;
     __shared__ __attribute__((aligned(16))) uint8_t s_t[TH][TS];
-    const int z = blockIdx.z & 1;          // view
-    const size_t pr = blockIdx.z >> 1;     // pair
+    const Tile3 tl = xcdTile();
+    const int z = tl.z & 1;          // view
+    const size_t pr = tl.z >> 1;     // pair
     const uint8_t* img = z ? pairAt(right, pr, b.right) : pairAt(left, pr, b.left);
     const int step = z ? rstep : lstep;
     cen = pairAt(cen, pr, b.ws);
     keys = pairAt(keys, pr, b.ws);
-    const int x0 = blockIdx.x * 64, y0 = blockIdx.y * kCenRows;
+    const int x0 = tl.x * 64, y0 = tl.y * kCenRows;
     uint32_t v[NL];
 #pragma unroll
     for (int k = 0; k < NL; ++k) {
@@ -150,10 +151,11 @@ __global__ __launch_bounds__(256) void kPrepBT(const uint8_t* left, int lstep, c
                                                int rstep, uint2* desc, uint32_t* keys, int W, int H,
                                                int cap, Batch b)
 {
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int z = blockIdx.z & 1;          // view
-    const size_t pr = blockIdx.z >> 1;     // pair
+    const Tile3 tl = xcdTile();
+    const int x = tl.x * 64 + (threadIdx.x & 63);
+    const int y = tl.y * 4 + (threadIdx.x >> 6);
+    const int z = tl.z & 1;          // view
+    const size_t pr = tl.z >> 1;     // pair
     if (x >= W || y >= H) return;
     const uint8_t* img = z ? pairAt(right, pr, b.right) : pairAt(left, pr, b.left);
     const int step = z ? rstep : lstep;
@@ -218,13 +220,14 @@ __global__ __launch_bounds__(256) void kCensusCost(const uint32_t* cen, int16_t*
                                                    int D, int Dp, int minD, int maxD, int geomMask,
                                                    size_t wsStride)
 {
+    const Tile3 tl = xcdTile();
     extern __shared__ uint32_t s_r[];
-    cen = pairAt(cen, blockIdx.z, wsStride);
-    C = pairAt(C, blockIdx.z, wsStride);
+    cen = pairAt(cen, tl.z, wsStride);
+    C = pairAt(C, tl.z, wsStride);
     const int tpp = Dp >> 3;
     const int npx = 256 / tpp;
-    const int y = blockIdx.y;
-    const int x0 = blockIdx.x * npx;
+    const int y = tl.y;
+    const int x0 = tl.x * npx;
     const int base2 = x0 - minD - (Dp - 1);
     const int nR = npx + Dp - 1;
     const uint32_t* L = cen + (size_t)y * W;
@@ -348,9 +351,10 @@ __global__ __launch_bounds__(256) void kCensusLocal(const uint32_t* cen, RefineI
                                                     int W, int H, int D, int Dp, int minD, int maxD,
                                                     Batch b)
 {
+    const Tile3 tl = xcdTile();
     constexpr int NPRE = 2;  // staged words per thread and row loaded ahead
     extern __shared__ uint32_t s_r[];
-    const size_t pr = blockIdx.z;
+    const size_t pr = tl.z;
     cen = pairAt(cen, pr, b.ws);
     labels = pairAt(labels, pr, b.ws);
     C = pairAt(C, pr, b.ws);
@@ -359,8 +363,8 @@ __global__ __launch_bounds__(256) void kCensusLocal(const uint32_t* cen, RefineI
     r.pconf = pairAt(r.pconf, pr, b.pconf);
     const int tpp = Dp >> 3;
     const int npx = 256 / tpp;
-    const int y0 = blockIdx.y * ROWS;
-    const int x0 = blockIdx.x * npx;
+    const int y0 = tl.y * ROWS;
+    const int x0 = tl.x * npx;
     const int xb = x0 - maxD;  // staged column 0
     const int span = npx + maxD - minD;
     const int px = threadIdx.x / tpp;
@@ -736,6 +740,7 @@ __global__ __launch_bounds__(256) void kBTLocalTile(const uint2* desc, RefineIn 
                                                     const uint8_t* img, int istep, int16_t* C,
                                                     int W, int H, int minD, int maxD, Batch b)
 {
+    const Tile3 tl = xcdTile();
     typedef uint16_t u2 __attribute__((ext_vector_type(2)));
     constexpr int NC = kTilePx + 2 * HALF;       // window columns of the block
     constexpr int NY = 2 * HALF + 1;             // window rows of one output row
@@ -748,14 +753,14 @@ __global__ __launch_bounds__(256) void kBTLocalTile(const uint2* desc, RefineIn 
     __shared__ uint32_t s_v[kTileR][NC];
     __shared__ int s_base[kTileRows][kTilePx];
     __shared__ int s_lo[kTileRows], s_hi[kTileRows];
-    const size_t pr = blockIdx.z;
+    const size_t pr = tl.z;
     desc = pairAt(desc, pr, b.ws);
     labels = pairAt(labels, pr, b.ws);
     C = pairAt(C, pr, b.ws);
     img = pairAt(img, pr, b.left);
     r.pdisp = pairAt(r.pdisp, pr, b.pdisp);
     r.pconf = pairAt(r.pconf, pr, b.pconf);
-    const int y0 = blockIdx.y * kTileRows, x0 = blockIdx.x * kTilePx, tid = threadIdx.x;
+    const int y0 = tl.y * kTileRows, x0 = tl.x * kTilePx, tid = threadIdx.x;
     const uint2* Ld = desc;
     const uint2* Rd = desc + (size_t)W * H;
     // Label windows: wave w holds row y0 + w.
@@ -929,13 +934,14 @@ __global__ __launch_bounds__(256) void kPyrDown(const uint8_t* srcA, int sstepA,
                                                 uint8_t* dstA, int dstepA, uint8_t* dstB,
                                                 int dstepB, size_t dStride, int dw, int dh, int nv)
 {
+    const Tile3 tl = xcdTile();
     constexpr int TW = 131, TH = 2 * kPyrRows + 3, TILE = TW * TH, NL = (TILE + 255) / 256;
     __shared__ uint8_t s_t[TH][TW + 1];
-    const size_t pr = blockIdx.z / nv;              // pair
-    const int view = (int)(blockIdx.z - pr * nv);   // 0: A, 1: B
+    const size_t pr = tl.z / nv;              // pair
+    const int view = (int)(tl.z - pr * nv);   // 0: A, 1: B
     const uint8_t* src = pairAt(view ? srcB : srcA, pr, sStride);
     const int sstep = view ? sstepB : sstepA;
-    const int x0 = blockIdx.x * 64, y0 = blockIdx.y * kPyrRows;
+    const int x0 = tl.x * 64, y0 = tl.y * kPyrRows;
     uint32_t v[NL];
 #pragma unroll
     for (int k = 0; k < NL; ++k) {

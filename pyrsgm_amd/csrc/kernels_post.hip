@@ -103,10 +103,11 @@ __device__ __forceinline__ int median9(int p[9])
 
 __global__ __launch_bounds__(256) void kPost(PostArgs args)
 {
+    const Tile3 tl = xcdTile();
     constexpr int HW = kPostW + 2, HALO = HW * (kPostH + 2), NH = (HALO + 255) / 256;
-    // This pair's buffers (pair = blockIdx.z).
+    // This pair's buffers (pair = tl.z).
     PostArgs a = args;
-    const size_t pr = blockIdx.z;
+    const size_t pr = tl.z;
     a.disp = pairAt(a.disp, pr, a.batch.ws);
     a.conf = pairAt(a.conf, pr, a.batch.ws);
     a.keys = pairAt(a.keys, pr, a.batch.ws);
@@ -116,7 +117,7 @@ __global__ __launch_bounds__(256) void kPost(PostArgs args)
     __shared__ int16_t s_d[kPostH + 2][HW];
     const int invalid = (int)(int16_t)((a.minD - 1) * 16);
     const int lx = threadIdx.x & 63, ly0 = threadIdx.x >> 6;
-    const int bx = blockIdx.x * kPostW, by = blockIdx.y * kPostH;
+    const int bx = tl.x * kPostW, by = tl.y * kPostH;
     const int x = bx + lx;
     const int xc = imin(x, a.W - 1);
     // Own pixels: raw disparity and confidence.

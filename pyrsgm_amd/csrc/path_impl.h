@@ -1154,20 +1154,12 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
     }
 }
 
-// Workgroups are dispatched to the 8 XCDs round-robin by linear index, and
-// each XCD has its own L2.  T->B block b of n runs column group
-// xcdRemap(b, n): the blocks of one XCD (equal b % 8) get a contiguous range
-// of groups, so that neighbouring column groups, which read parts of the same
-// 128-byte lines at every step, share one L2 instead of each XCD fetching the
-// line.  A bijection of [0, n).  (Row paths read whole lines per chunk and keep
+// T->B block b of n runs column group xcdRemap(b, n): the blocks of one XCD
+// get a contiguous range of groups, so that neighbouring column groups, which
+// read parts of the same 128-byte lines at every step, share one L2 instead of
+// each XCD fetching the line.  (Row paths read whole lines per chunk and keep
 // the dispatch order, which spreads their long lines evenly over the XCDs.)
-__device__ __forceinline__ int xcdRemap(int b, int n)
-{
-    const int q = n >> 3, r = n & 7, x = b & 7, i = b >> 3;
-    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
-}
 
-// MOD: 16-label byte-volume refinement lines use the modular step.
 template <int G, int K, int MODE, bool ADAPT, bool EXACT, bool NARROW, bool PK = false,
           bool MOD = false>
 __global__ __launch_bounds__(128) void kPaths(PathArgs a)

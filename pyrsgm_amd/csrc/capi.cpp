@@ -220,6 +220,13 @@ hipError_t launchPathsQueued(const QueueArgs& q, int mode, int Dp, int adaptive,
     return hipLaunchKernel(k, dim3(grid), dim3(128), params, 0, s);
 }
 
+int fusedWTA(const LevelCall& c, int exact)
+{
+    if (c.sink || !(c.stages & kStagePaths) || !(c.stages & kStageWTA)) return 0;
+    if (c.mode == kSingle || !c.narrow || exact || c.Dp != 16) return 0;
+    return c.tiles.fused == 2 ? 1 : 0;
+}
+
 hipError_t launchLevel(const LevelCall& c, Workspace& ws, hipStream_t s)
 {
     hipError_t e = hipSuccess;
@@ -268,12 +275,21 @@ hipError_t launchLevel(const LevelCall& c, Workspace& ws, hipStream_t s)
     static const int remap = getenv("PYRSGM_NO_XCD_REMAP") ? 0 : 1;
     a.xcdRemap = c.tiles.xcdRemap == 0 ? remap : (c.tiles.xcdRemap == 2 ? 1 : 0);
     a.pairedMode = c.tiles.paired;
-    a.modularMode = c.tiles.modular;
+    // PYRSGM_MODULAR=1 / 2 turns the modular refinement step off / on where the
+    // launch shape leaves it to the launch size (A/B timing; same results).
+    static const int modularEnv = [] {
+        const char* e = getenv("PYRSGM_MODULAR");
+        const int v = e ? atoi(e) : 0;
+        return (v == 1 || v == 2) ? v : 0;
+    }();
+    a.modularMode = c.tiles.modular ? c.tiles.modular : modularEnv;
+    a.fuseWTA = fusedWTA(c, a.exact);
     a.disp = ws.disp;
     a.conf = ws.conf;
     a.keys = ws.keys;
     a.uniq = c.uniq;
     a.minConf = c.minConf;
+    a.invW = std::nextafter(1.0 / c.W, 2.0);
     a.batch = c.batch;
     memcpy(a.lut, c.lut, sizeof(a.lut));
 
@@ -287,7 +303,7 @@ hipError_t launchLevel(const LevelCall& c, Workspace& ws, hipStream_t s)
             if (e != hipSuccess) return e;
         }
     }
-    if (c.stages & kStageWTA) {
+    if ((c.stages & kStageWTA) && !a.fuseWTA) {
         profBegin(PYRSGM_STAGE_WTA, c.W, c.H, c.D, s);
         e = launchWTA(a, c.mode, s);
         profEnd(s);

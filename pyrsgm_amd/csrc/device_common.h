@@ -167,6 +167,52 @@ __device__ __forceinline__ uint32_t quad_umin(uint32_t v)
     return v;
 }
 
+// Row-internal DPP moves without a fill value fold into the consuming
+// instruction (v_min_i32_dpp).
+template <int CTRL>
+__device__ __forceinline__ int dmov(int v)
+{
+    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, true);
+}
+
+// Minimum over each group of G consecutive lanes (G in 1..64, power of two),
+// returned to every lane of the group.
+template <int G>
+__device__ __forceinline__ int gmin(int v)
+{
+    if constexpr (G >= 2) v = imin(v, dmov<kDppQuadXor1>(v));
+    if constexpr (G >= 4) v = imin(v, dmov<kDppQuadXor2>(v));
+    if constexpr (G >= 8) v = imin(v, dmov<kDppRowHalfMirror>(v));
+    if constexpr (G >= 16) v = imin(v, dmov<kDppRowMirror>(v));
+    if constexpr (G >= 32) {
+        const auto s = __builtin_amdgcn_permlane16_swap((uint32_t)v, (uint32_t)v, false, false);
+        v = imin((int)s[0], (int)s[1]);
+    }
+    if constexpr (G >= 64) {
+        const auto s = __builtin_amdgcn_permlane32_swap((uint32_t)v, (uint32_t)v, false, false);
+        v = imin((int)s[0], (int)s[1]);
+    }
+    return v;
+}
+
+template <int G>
+__device__ __forceinline__ uint32_t gumin(uint32_t v)
+{
+    if constexpr (G >= 2) v = umin(v, (uint32_t)dmov<kDppQuadXor1>((int)v));
+    if constexpr (G >= 4) v = umin(v, (uint32_t)dmov<kDppQuadXor2>((int)v));
+    if constexpr (G >= 8) v = umin(v, (uint32_t)dmov<kDppRowHalfMirror>((int)v));
+    if constexpr (G >= 16) v = umin(v, (uint32_t)dmov<kDppRowMirror>((int)v));
+    if constexpr (G >= 32) {
+        const auto s = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        v = umin(s[0], s[1]);
+    }
+    if constexpr (G >= 64) {
+        const auto s = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+        v = umin(s[0], s[1]);
+    }
+    return v;
+}
+
 // ---- K consecutive int16 per lane, kept packed two per dword ------------
 template <int K>
 struct Packed {

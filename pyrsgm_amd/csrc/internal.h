@@ -77,6 +77,7 @@ struct TileConfig {
     int paired = 0;     // paired lines for 16-label byte volumes: 0 by launch size, 1 off, 2 on
     int xcdRemap = 0;   // XCD-contiguous T->B column groups: 0 default, 1 off, 2 on
     int modular = 0;    // modular refinement step: 0 by launch size, 1 off, 2 on
+    int fused = 0;      // R->L lines finish the winner-take-all: 0 default (off), 1 off, 2 on
 };
 
 // Everything one level needs; filled by the C-ABI entry points.
@@ -139,6 +140,7 @@ struct PathArgs {
     int xcdRemap;              // XCD-contiguous line groups (on unless PYRSGM_NO_XCD_REMAP is set)
     int pairedMode;            // TileConfig::paired
     int modularMode;           // TileConfig::modular
+    int fuseWTA;               // the R->L lines finish the winner-take-all (see fusedWTA)
     int nLRBlocks;             // combined launch: blocks [0, nLR) run L->R,
     int nRLBlocks;             // [nLR, nLR+nRL) R->L, the rest T->B
     int16_t* disp;
@@ -146,6 +148,7 @@ struct PathArgs {
     uint32_t* keys;
     int uniq;
     float minConf;
+    double invW;               // 1/W rounded up by one ulp (pixel rows in the winner-take-all)
     Batch batch;
     int32_t lut[256];
 };
@@ -209,6 +212,17 @@ int pairedLinesAuto(size_t blocks4);
 // Path launches of at least this many 4-line waves use paired lines (byte
 // volumes, 16 labels, pyramid levels): four per SIMD.
 constexpr size_t kPairedWaves = 4096;
+// Fused winner-take-all (TileConfig::fused = 2, 16-label byte-volume pyramid
+// levels in the fast form): the L->R and T->B lines run in one launch, then
+// the R->L lines finish each pixel from its three path-cost vectors, so the
+// R->L volume is neither written nor read again.  It is a tuning choice, off
+// unless asked for: in launches large enough to be bound by VALU issue the
+// winner-take-all's instructions move into the R->L launch unchanged, and
+// smaller launches are bound by the recurrence's latency, which running R->L
+// after the other two directions lengthens.
+// Whether a level call runs the fused winner-take-all (TileConfig::fused,
+// volume form and stages).
+int fusedWTA(const LevelCall& c, int exact);
 // Per-launch 16-label byte-volume refinement path launches of at most this many
 // workgroups use the modular step (measured: faster at 510, slower at 1020).
 constexpr size_t kModularMaxBlocks = 768;

@@ -77,22 +77,24 @@ pyrsgm_params_c fromParams(const pyrsgm::Params& p)
 } // namespace
 
 namespace {
-pyrsgm::BatchTuning batchTuning(const int t[4])
+pyrsgm::BatchTuning batchTuning(const int t[5])
 {
     pyrsgm::BatchTuning b;
     b.census_rows = t[0];
     b.paired = t[1];
     b.xcd_remap = t[2];
     b.side_stream = t[3];
+    b.fused = t[4];
     return b;
 }
 
-void batchTuningOut(const pyrsgm::BatchTuning& b, int t[4])
+void batchTuningOut(const pyrsgm::BatchTuning& b, int t[5])
 {
     t[0] = b.census_rows;
     t[1] = b.paired;
     t[2] = b.xcd_remap;
     t[3] = b.side_stream;
+    t[4] = b.fused;
 }
 } // namespace
 
@@ -258,7 +260,7 @@ int pyrsgm_batch_launch(void* matcher, hipStream_t stream, char* err, int errlen
 }
 
 
-int pyrsgm_batch_set_tuning(void* matcher, const int tuning[4], char* err, int errlen)
+int pyrsgm_batch_set_tuning(void* matcher, const int tuning[5], char* err, int errlen)
 {
     if (!matcher || !tuning) return -1;
     try {
@@ -270,7 +272,7 @@ int pyrsgm_batch_set_tuning(void* matcher, const int tuning[4], char* err, int e
     return -1;
 }
 
-int pyrsgm_batch_get_tuning(void* matcher, int tuning[4])
+int pyrsgm_batch_get_tuning(void* matcher, int tuning[5])
 {
     if (!matcher || !tuning) return -1;
     batchTuningOut(static_cast<pyrsgm::BatchMatcher*>(matcher)->tuning(), tuning);
@@ -283,7 +285,7 @@ int pyrsgm_batch_tune(void* matcher,
                       short* d_disp16, size_t disp16_pitch, size_t disp16_stride,
                       float* d_disp32, size_t disp32_pitch, size_t disp32_stride,
                       float* d_conf, size_t conf_pitch, size_t conf_stride,
-                      hipStream_t stream, int reps, int best[4], double* best_ms, char* err,
+                      hipStream_t stream, int reps, int best[5], double* best_ms, char* err,
                       int errlen)
 {
     if (!matcher) return -1;

@@ -37,6 +37,7 @@
 
 #include "device_common.h"
 #include "internal.h"
+#include "wta_impl.h"
 
 namespace pyr {
 namespace path {
@@ -152,52 +153,6 @@ __device__ __forceinline__ int unpk(const uint32_t (&w)[(K + 1) / 2], int k)
 {
     const uint32_t x = w[k >> 1];
     return (int)((k & 1) ? (x >> 16) : (x & 0xFFFFu));
-}
-
-// Row-internal DPP moves without a fill value fold into the consuming
-// instruction (v_min_i32_dpp).
-template <int CTRL>
-__device__ __forceinline__ int dmov(int v)
-{
-    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, true);
-}
-
-// Minimum over each group of G consecutive lanes (G in 1..64, power of two),
-// returned to every lane of the group.
-template <int G>
-__device__ __forceinline__ int gmin(int v)
-{
-    if constexpr (G >= 2) v = imin(v, dmov<kDppQuadXor1>(v));
-    if constexpr (G >= 4) v = imin(v, dmov<kDppQuadXor2>(v));
-    if constexpr (G >= 8) v = imin(v, dmov<kDppRowHalfMirror>(v));
-    if constexpr (G >= 16) v = imin(v, dmov<kDppRowMirror>(v));
-    if constexpr (G >= 32) {
-        const auto s = __builtin_amdgcn_permlane16_swap((uint32_t)v, (uint32_t)v, false, false);
-        v = imin((int)s[0], (int)s[1]);
-    }
-    if constexpr (G >= 64) {
-        const auto s = __builtin_amdgcn_permlane32_swap((uint32_t)v, (uint32_t)v, false, false);
-        v = imin((int)s[0], (int)s[1]);
-    }
-    return v;
-}
-
-template <int G>
-__device__ __forceinline__ uint32_t gumin(uint32_t v)
-{
-    if constexpr (G >= 2) v = umin(v, (uint32_t)dmov<kDppQuadXor1>((int)v));
-    if constexpr (G >= 4) v = umin(v, (uint32_t)dmov<kDppQuadXor2>((int)v));
-    if constexpr (G >= 8) v = umin(v, (uint32_t)dmov<kDppRowHalfMirror>((int)v));
-    if constexpr (G >= 16) v = umin(v, (uint32_t)dmov<kDppRowMirror>((int)v));
-    if constexpr (G >= 32) {
-        const auto s = __builtin_amdgcn_permlane16_swap(v, v, false, false);
-        v = umin(s[0], s[1]);
-    }
-    if constexpr (G >= 64) {
-        const auto s = __builtin_amdgcn_permlane32_swap(v, v, false, false);
-        v = umin(s[0], s[1]);
-    }
-    return v;
 }
 
 // Maximum over each group of G consecutive lanes (G in 16..64), returned to
@@ -468,7 +423,7 @@ __device__ __forceinline__ int andByteMask(int x, uint32_t m)
 
 // MODE: kSingle, kCoarse or kRefine.
 template <int G, int K, int MODE, bool ADAPT, bool EXACT, bool NARROW, bool PK, int DIR,
-          bool MOD = false>
+          bool MOD = false, bool FUSE = false>
 __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int lineGroup,
                                         const int* s_lut, int* s_xchg, char* s_lds)
 {
@@ -485,6 +440,10 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
     // Modular refinement step: compute lane g holds the label whose real
     // disparity is congruent to g modulo 16 (see the step below).
     constexpr bool MODR = modularRefine<G, K, MODE, EXACT, NARROW, PK, MOD>();
+    // FUSE: R->L lines of a pyramid level whose winner-take-all is done here
+    // (the L->R and T->B volumes were written by an earlier launch).
+    static_assert(!FUSE || (RL && PYR && NARROW && G == 16 && K == 1 && !EXACT),
+                  "the fused winner-take-all runs on 16-label byte-volume R->L lines");
     using TL = Lds<G, K, NARROW, PK, MODR>;
     constexpr int ESZ = TL::ESZ;
     constexpr int VB = GK * ESZ;  // bytes of one pixel's label vector
@@ -717,34 +676,129 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
         }
     };
 
+    // ---- fused winner-take-all (FUSE) -----------------------------------------
+    // Instead of storing tile k, the memory wave finishes its pixels: piece pc
+    // of the tile is one pixel's 16 R->L costs (row pc / PPR = line, column
+    // (pc % PPR) in address order), and the pixel's L->R and T->B vectors and
+    // label word are loaded into register set k % 2 two iterations earlier.
+    struct ABuf {
+        u4v l[FUSE ? TL::NQ : 1], t[FUSE ? TL::NQ : 1];
+        int w[FUSE ? TL::NQ : 1];
+    };
+    auto aload = [&](ABuf& b, int k) {
+        if constexpr (FUSE) {
+            const int sc = chunkStart(k);
+            const bool full = chunkFull(k);
+            const int nSteps = imin(U, s1 - sc);
+            const ChunkAddr<U> co = chunk(sc, volRow, VB, VB, full);
+            const auto rL = rsrc(reinterpret_cast<const char*>(pairAt(a.Alr, pair, a.batch.ws)) + co.origin, co.nr);
+            const auto rT = rsrc(reinterpret_cast<const char*>(pairAt(a.Atb, pair, a.batch.ws)) + co.origin, co.nr);
+            const u4v none = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};  // path not run
+#pragma unroll
+            for (int i = 0; i < TL::NQ; ++i) {
+                const int pc = lane + 64 * i;
+                const int row = pc / PPR;
+                const int ob = (pc - row * PPR) * 16;
+                const bool ok = row < nValid && ob / VB < nSteps;
+                const uint32_t go = ok ? (uint32_t)row * volRow + (uint32_t)ob : kOffLimit;
+                b.l[i] = a.numPaths >= 2 ? __builtin_amdgcn_raw_buffer_load_b128(rL, go, 0, 0) : none;
+                b.t[i] = a.numPaths >= 3 ? __builtin_amdgcn_raw_buffer_load_b128(rT, go, 0, 0) : none;
+                if constexpr (REFINE) {
+                    const ChunkAddr<U> cw = chunk(sc, (uint32_t)W * 4, 4, 4, full);
+                    const auto rW = rsrc(reinterpret_cast<const char*>(labelsV) + cw.origin, cw.nr);
+                    const uint32_t wo = ok ? (uint32_t)row * (uint32_t)W * 4 + (uint32_t)(ob / VB) * 4 : kOffLimit;
+                    b.w[i] = (int)__builtin_amdgcn_raw_buffer_load_b32(rW, wo, 0, 0);
+                } else {
+                    b.w[i] = 0;
+                }
+            }
+        } else {
+            (void)b;
+            (void)k;
+        }
+    };
+    auto wtaTile = [&](const ABuf& b, int k) {
+        if constexpr (FUSE) {
+            const int sc = chunkStart(k);
+            const bool full = chunkFull(k);
+            const int nSteps = imin(U, s1 - sc);
+            const int sLast = full ? sc + U - 1 : imin(sc + U - 1, s1 - 1);
+            const int col0 = W - 1 - sLast;  // the chunk's lowest column (R->L runs right to left)
+            const char* const tile = s_lds + TL::OFF_T + (k & 1) * TL::TSLOT;
+            int16_t* const dispV = pairAt(a.disp, pair, a.batch.ws);
+            float* const confV = pairAt(a.conf, pair, a.batch.ws);
+            uint32_t* const keysV = pairAt(a.keys, pair, a.batch.ws);
+#pragma unroll
+            for (int i = 0; i < TL::NQ; ++i) {
+                const int pc = lane + 64 * i;
+                const int row = pc / PPR;
+                const int ob = (pc - row * PPR) * 16;
+                u4v v = *reinterpret_cast<const u4v*>(tile + row * TL::ROWH + ob);
+                if constexpr (MODR) {
+                    const int rot = *reinterpret_cast<const uint8_t*>(s_lds + TL::OFF_R + (k & 3) * 64 + pc);
+                    v = rotDown16(v, rot);
+                }
+                const int col = ob / VB;
+                if (row < nValid && col < nSteps) {
+                    const int x = col0 + col, y = line0 + row;
+                    const uint32_t wr[4] = {v[0], v[1], v[2], v[3]};
+                    const uint32_t wl[4] = {b.l[i][0], b.l[i][1], b.l[i][2], b.l[i][3]};
+                    const uint32_t wt[4] = {b.t[i][0], b.t[i][1], b.t[i][2], b.t[i][3]};
+                    const wta::Pick pk = wta::searchNarrow<1>(wr, wl, wt, 0, D);
+                    wta::resolve<MODE>(a, pk, x, y, (size_t)y * W + x, b.w[i], dispV, confV, keysV);
+                }
+            }
+        } else {
+            (void)b;
+            (void)k;
+        }
+    };
+    // Tile k leaves the workgroup: stored, or (FUSE) finished from set k % 2.
+    auto finish = [&](const ABuf& A0, const ABuf& A1, int k) {
+        if constexpr (FUSE) {
+            if (k & 1) wtaTile(A1, k);
+            else wtaTile(A0, k);
+        } else {
+            (void)A0;
+            (void)A1;
+            flushTile(k);
+        }
+    };
+
     if (memWave) {
         // Chunk c's loads are issued two iterations before it is scattered
         // into LDS (register sets R0 / R1 by chunk parity); iteration k
         // (between barriers k and k+1) fills slot k+2 and stores tile k-1.
+        // FUSE: iteration k finishes chunk k-1 from set (k-1) % 2 and then
+        // loads chunk k+1's vectors into that set.
         MBuf R0, R1;
+        ABuf A0, A1;
         mload(R0, 0);
         if (nChunks > 1) mload(R1, 1);
         mstore(R0, 0);
         if (nChunks > 1) mstore(R1, 1);
         if (nChunks > 2) mload(R0, 2);
         if (nChunks > 3) mload(R1, 3);
+        aload(A0, 0);
         wgBarrier();
         int k = 0;
         for (; k + 1 < nChunks; k += 2) {
             if (k + 2 < nChunks) mstore(R0, k + 2);
             if (k + 4 < nChunks) mload(R0, k + 4);
-            if (k >= 1) flushTile(k - 1);
+            if (k >= 1) finish(A0, A1, k - 1);
+            aload(A1, k + 1);
             wgBarrier();
             if (k + 3 < nChunks) mstore(R1, k + 3);
             if (k + 5 < nChunks) mload(R1, k + 5);
-            flushTile(k);
+            finish(A0, A1, k);
+            if (k + 2 < nChunks) aload(A0, k + 2);
             wgBarrier();
         }
         if (k < nChunks) {
-            if (k >= 1) flushTile(k - 1);
+            if (k >= 1) finish(A0, A1, k - 1);
             wgBarrier();
         }
-        flushTile(nChunks - 1);
+        finish(A0, A1, nChunks - 1);
         return;
     }
 
@@ -1186,6 +1240,24 @@ __global__ __launch_bounds__(128) void kPaths(PathArgs a)
     }
 }
 
+// R->L lines of a 16-label byte-volume pyramid level with the winner-take-all
+// fused in (after a launch of the level's L->R and T->B lines): block b runs
+// line group b of pair blockIdx.y.  PK: paired lines; MOD: modular step.
+template <int MODE, bool ADAPT, bool PK, bool MOD>
+__global__ __launch_bounds__(128) void kPathsRLW(PathArgs a)
+{
+    __shared__ int s_lut[256];
+    __shared__ int s_xchg[MODE == kRefine ? 64 : 1];
+    __shared__ __attribute__((aligned(16))) char
+        s_lds[Lds<16, 1, true, PK, modularRefine<16, 1, MODE, false, true, PK, MOD>()>::BYTES];
+    if constexpr (ADAPT) {
+        for (int i = threadIdx.x; i < 256; i += 128) s_lut[i] = a.lut[i];
+        __syncthreads();
+    }
+    runLine<16, 1, MODE, ADAPT, false, true, PK, kRL, MOD, true>(a, blockIdx.y, blockIdx.x, s_lut, s_xchg,
+                                                                s_lds);
+}
+
 // Queued form: persistent workgroups take (task, block) items from a device
 // counter until the items run out; a task is one pair's level with its own
 // geometry, penalties and buffers, numbered blocks as in kPaths.  Tasks are
@@ -1245,15 +1317,17 @@ __global__ __launch_bounds__(128) void kPathsQ(QueueArgs q)
 inline int linesPerWave(int Dp) { return Dp <= 16 ? 4 : (Dp <= 32 ? 2 : 1); }
 
 
+// The L->R / T->B / R->L lines of one launch (withRL: R->L lines included).
 template <int MODE, bool ADAPT, bool EXACT, bool NARROW>
-hipError_t dispatchPaths(const PathArgs& a, hipStream_t s)
+hipError_t launchDirs(const PathArgs& a, bool withRL, hipStream_t s)
 {
     const int lpw = linesPerWave(a.Dp);
     PathArgs b = a;
     b.nLRBlocks = a.numPaths >= 2 ? (a.H + lpw - 1) / lpw : 0;
-    b.nRLBlocks = (a.H + lpw - 1) / lpw;
+    b.nRLBlocks = withRL ? (a.H + lpw - 1) / lpw : 0;
     const int nTB = a.numPaths >= 3 ? (a.W + lpw - 1) / lpw : 0;
     const int blocks = b.nLRBlocks + b.nRLBlocks + nTB;
+    if (blocks == 0) return hipSuccess;
     const dim3 grid(blocks, a.batch.n);
     if constexpr (NARROW && MODE != kSingle) {
         // Paired lines when the launch has at least four 4-line waves per SIMD
@@ -1263,7 +1337,7 @@ hipError_t dispatchPaths(const PathArgs& a, hipStream_t s)
                            (a.pairedMode == 0 && (size_t)blocks * a.batch.n >= kPairedWaves))) {
             PathArgs c = a;
             c.nLRBlocks = a.numPaths >= 2 ? (a.H + 7) / 8 : 0;
-            c.nRLBlocks = (a.H + 7) / 8;
+            c.nRLBlocks = withRL ? (a.H + 7) / 8 : 0;
             const int nTB8 = a.numPaths >= 3 ? (a.W + 7) / 8 : 0;
             const dim3 grid8(c.nLRBlocks + c.nRLBlocks + nTB8, a.batch.n);
             kPaths<16, 1, MODE, ADAPT, EXACT, true, true><<<grid8, 128, 0, s>>>(c);
@@ -1309,6 +1383,44 @@ hipError_t dispatchPaths(const PathArgs& a, hipStream_t s)
         return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+// R->L lines with the fused winner-take-all; paired lines and the modular
+// step are chosen by this launch's own size, as in launchDirs.
+template <int MODE, bool ADAPT>
+hipError_t launchRLW(const PathArgs& a, hipStream_t s)
+{
+    const size_t rl4 = (size_t)((a.H + 3) / 4) * a.batch.n;
+    if (a.pairedMode == 2 || (a.pairedMode == 0 && rl4 >= kPairedWaves)) {
+        kPathsRLW<MODE, ADAPT, true, false><<<dim3((a.H + 7) / 8, a.batch.n), 128, 0, s>>>(a);
+        return hipGetLastError();
+    }
+    const dim3 grid((a.H + 3) / 4, a.batch.n);
+    if constexpr (MODE == kRefine) {
+        if (a.modularMode == 2 || (a.modularMode == 0 && rl4 <= kModularMaxBlocks)) {
+            kPathsRLW<MODE, ADAPT, false, true><<<grid, 128, 0, s>>>(a);
+            return hipGetLastError();
+        }
+    }
+    kPathsRLW<MODE, ADAPT, false, false><<<grid, 128, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+template <int MODE, bool ADAPT, bool EXACT, bool NARROW>
+hipError_t dispatchPaths(const PathArgs& a, hipStream_t s)
+{
+    // Fused winner-take-all (the host sets fuseWTA only for 16-label
+    // byte-volume pyramid levels in the fast form): the L->R and T->B lines
+    // run first, then the R->L lines, which finish every pixel themselves.
+    if constexpr (NARROW && MODE != kSingle && !EXACT) {
+        if (a.fuseWTA) {
+            if (a.Dp != 16) return hipErrorInvalidValue;
+            const hipError_t e = launchDirs<MODE, ADAPT, EXACT, NARROW>(a, false, s);
+            if (e != hipSuccess) return e;
+            return launchRLW<MODE, ADAPT>(a, s);
+        }
+    }
+    return launchDirs<MODE, ADAPT, EXACT, NARROW>(a, true, s);
 }
 
 // Queued launches: the kernel of one specialisation, its occupancy and launch.

@@ -126,6 +126,8 @@ void checkBatchTuning(const BatchTuning& t)
         throw std::invalid_argument("pyrsgm: BatchTuning.paired / xcd_remap must be -1, 0 or 1");
     if (t.side_stream != 0 && t.side_stream != 1)
         throw std::invalid_argument("pyrsgm: BatchTuning.side_stream must be 0 or 1");
+    if (t.fused < -1 || t.fused > 1)
+        throw std::invalid_argument("pyrsgm: BatchTuning.fused must be -1, 0 or 1");
 }
 
 } // namespace
@@ -153,6 +155,7 @@ struct BatchMatcher::Impl : MatcherCore {
         tiles.paired = t.paired < 0 ? 0 : (t.paired ? 2 : 1);
         tiles.xcdRemap = t.xcd_remap < 0 ? 0 : (t.xcd_remap ? 2 : 1);
         useSide = t.side_stream != 0;
+        tiles.fused = t.fused < 0 ? 0 : (t.fused ? 2 : 1);
     }
 
     double timePasses(BatchBuffer<const uint8_t> L, BatchBuffer<const uint8_t> R,
@@ -197,6 +200,8 @@ struct BatchMatcher::Impl : MatcherCore {
             if (pk != best.paired) { BatchTuning t = best; t.paired = pk; consider(t); }
         for (int xr : {-1, 0, 1})
             if (xr != best.xcd_remap) { BatchTuning t = best; t.xcd_remap = xr; consider(t); }
+        for (int fu : {-1, 0, 1})
+            if (fu != best.fused) { BatchTuning t = best; t.fused = fu; consider(t); }
         if (side)
             for (int sd : {0, 1})
                 if (sd != best.side_stream) { BatchTuning t = best; t.side_stream = sd; consider(t); }

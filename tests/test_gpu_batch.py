@@ -191,20 +191,22 @@ def test_batch_paired_lines_equal_single_pairs(n, w, h, D, levels):
     U.assert_same("pair 1 vs oracle", d16[1].cpu().numpy(), ref.disp16)
 
 
-@pytest.mark.parametrize("census_rows,paired,xcd_remap,side_stream",
-                         [(2, 1, 0, 0), (8, 0, 1, 1), (4, -1, -1, 1), (2, -1, 1, 0)])
-def test_batch_tunings_equal_single_pairs(census_rows, paired, xcd_remap, side_stream):
+@pytest.mark.parametrize("census_rows,paired,xcd_remap,side_stream,fused",
+                         [(2, 1, 0, 0, 1), (8, 0, 1, 1, 1), (4, -1, -1, 1, -1), (2, -1, 1, 0, 0),
+                          (4, 1, -1, 1, 0), (8, -1, 0, 1, 1)])
+def test_batch_tunings_equal_single_pairs(census_rows, paired, xcd_remap, side_stream, fused):
     """Every launch shape (census rows per block, paired lines forced on or off,
-    XCD remap, side stream) gives the same bits as the pairs matched alone."""
+    XCD remap, side stream, winner-take-all fused into the R->L lines or not)
+    gives the same bits as the pairs matched alone."""
     n, w, h, D = 3, 241, 135, 128
     lefts, rights = pairs(n, w, h, D - 1, seed0=900)
     params = P.validate_params(P.Params(num_disparities=D, pyramid_levels=2,
                                         cost_mode=P.CostMode.Census, p1=8, p2=32,
                                         adaptive_p2=True, median_filter=True))
     bm = P.BatchMatcher(w, h, n, params)
-    bm.set_tuning(census_rows, paired, xcd_remap, side_stream)
+    bm.set_tuning(census_rows, paired, xcd_remap, side_stream, fused)
     assert bm.tuning() == {"census_rows": census_rows, "paired": paired, "xcd_remap": xcd_remap,
-                           "side_stream": side_stream}
+                           "side_stream": side_stream, "fused": fused}
     print("case: batch tuning", bm.tuning(), flush=True)
     d16, d32, cf = run_batch(bm, batch_tensor(lefts, torch.uint8), batch_tensor(rights, torch.uint8),
                              n, h, w)
@@ -212,6 +214,48 @@ def test_batch_tunings_equal_single_pairs(census_rows, paired, xcd_remap, side_s
     for i in range(n):
         U.assert_same(f"pair {i} disp16", d16[i].cpu().numpy(), want[i][0])
         U.assert_same(f"pair {i} conf", cf[i].cpu().numpy(), want[i][2])
+    bm.close()
+
+
+FUSED_CASES = [
+    # (n, w, h, D, levels, adaptive, num_paths, paired): the fused winner-take-all
+    # at refinement levels (modular, paired and plain R->L lines) and at a
+    # 16-label coarse level (D=64 over two halvings), fixed and adaptive P2,
+    # with fewer paths (no T->B / no L->R volume read).
+    (3, 241, 135, 128, 2, 1, 3, 0),
+    (3, 241, 135, 128, 2, 1, 3, 1),
+    (2, 203, 117, 64, 2, 0, 3, 0),
+    (2, 203, 117, 64, 2, 1, 3, 1),
+    (2, 160, 96, 64, 2, 1, 2, 0),
+    (2, 160, 96, 64, 1, 0, 1, 1),
+]
+
+
+@pytest.mark.parametrize("n,w,h,D,levels,adaptive,num_paths,paired", FUSED_CASES)
+def test_fused_wta_equals_single_pairs(n, w, h, D, levels, adaptive, num_paths, paired):
+    """Winner-take-all finished by the R->L lines (forced on) gives the same bits
+    as separate winner-take-all launches (the pairs matched alone); pair 0 is
+    also compared with the oracle."""
+    lefts, rights = pairs(n, w, h, D - 1, seed0=1700 + w + levels)
+    params = P.validate_params(P.Params(num_disparities=D, pyramid_levels=levels,
+                                        cost_mode=P.CostMode.Census, p1=8, p2=32,
+                                        adaptive_p2=bool(adaptive), median_filter=True,
+                                        num_paths=num_paths))
+    bm = P.BatchMatcher(w, h, n, params)
+    bm.set_tuning(4, paired, -1, 1, 1)
+    print(f"case: fused wta {n}x{w}x{h} D={D} levels={levels} adaptive={adaptive} "
+          f"paths={num_paths} paired={paired}", flush=True)
+    d16, d32, cf = run_batch(bm, batch_tensor(lefts, torch.uint8), batch_tensor(rights, torch.uint8),
+                             n, h, w)
+    want = singles(lefts, rights, params)
+    for i in range(n):
+        U.assert_same(f"pair {i} disp16", d16[i].cpu().numpy(), want[i][0])
+        U.assert_same(f"pair {i} disp32", d32[i].cpu().numpy(), want[i][1])
+        U.assert_same(f"pair {i} conf", cf[i].cpu().numpy(), want[i][2])
+    ref = O.matcher_compute(lefts[0], rights[0], params)
+    assert ref.rc == 0
+    U.assert_same("pair 0 disp16 vs oracle", d16[0].cpu().numpy(), ref.disp16)
+    U.assert_same("pair 0 conf vs oracle", cf[0].cpu().numpy(), ref.conf)
     bm.close()
 
 

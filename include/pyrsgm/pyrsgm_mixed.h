@@ -55,7 +55,7 @@ typedef struct {
  * (0: the kernel's occupancy); paired: two lines per lane for 16-label byte
  * volumes (-1: by launch size, 0: off, 1: on). */
 typedef struct {
-    int branches, blocks_per_cu, paired;
+    int branches, blocks_per_cu, paired, queued_stages;
 } pyrsgm_mixed_tuning_c;
 
 void* pyrsgm_mixed_create(int n, const int* widths, const int* heights,
@@ -72,9 +72,10 @@ int pyrsgm_mixed_get_tuning(void* batch, pyrsgm_mixed_tuning_c* out);
  * captured graph. */
 int pyrsgm_mixed_tune(void* batch, const pyrsgm_pair_io_c* io, hipStream_t stream, int reps,
                       pyrsgm_mixed_tuning_c* best, double* best_ms, char* err, int errlen);
-/* out[4] = {queued path launches (each followed by one queued winner-take-all
- * launch), tasks in them, work items in them, per-pair stage launches} */
-int pyrsgm_mixed_plan(void* batch, long long out[4]);
+/* out[6] = {queued path launches (each followed by one queued winner-take-all
+ * launch), tasks in them, work items in them, per-pair stage launches, queued
+ * stage launches (pyrDown, descriptors, costs, post), tasks in them} */
+int pyrsgm_mixed_plan(void* batch, long long out[6]);
 size_t pyrsgm_mixed_device_bytes(void* batch);
 
 #ifdef __cplusplus
@@ -93,9 +94,10 @@ struct PairSpec {
 };
 
 struct MixedTuning {
-    int branches = 4;
-    int blocks_per_cu = 0;
-    int paired = -1;
+    int branches = 4;       ///< streams of the per-pair stage launches (queued_stages = 0)
+    int blocks_per_cu = 0;  ///< resident workgroups per CU of the queued path launches (0: occupancy)
+    int paired = -1;        ///< paired lines: -1 by launch size, 0 off, 1 on
+    int queued_stages = 1;  ///< pyrDown, descriptors, costs and post as queued launches (1) or per pair (0)
 };
 
 class MixedBatch {
@@ -120,7 +122,8 @@ public:
     int pairs() const;
     const Params& params(int i) const;
     size_t deviceMemoryBytes() const;
-    /// {queued path launches, tasks, work items, per-pair launches}
+    /// {queued path launches, tasks, work items, per-pair stage launches,
+    ///  queued stage launches, stage tasks}
     std::vector<long long> plan() const;
 
 private:

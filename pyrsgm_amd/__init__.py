@@ -558,8 +558,10 @@ class _PairIOC(C.Structure):
 class MixedTuning(C.Structure):
     """pyrsgm_mixed_tuning_c: streams for the per-pair stages (1..4), resident
     workgroups per CU of the queued path launches (0: occupancy), paired lines
-    (-1 by launch size, 0 off, 1 on)."""
-    _fields_ = [("branches", C.c_int), ("blocks_per_cu", C.c_int), ("paired", C.c_int)]
+    (-1 by launch size, 0 off, 1 on), and whether pyrDown, descriptors, costs and
+    post run as queued launches over all pairs (1) or per pair (0)."""
+    _fields_ = [("branches", C.c_int), ("blocks_per_cu", C.c_int), ("paired", C.c_int),
+                ("queued_stages", C.c_int)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -637,9 +639,9 @@ class MixedBatch:
         if lib().pyrsgm_mixed_launch(self._h, s.cuda_stream, err, 512) != 0:
             raise RuntimeError(err.value.decode())
 
-    def set_tuning(self, branches=4, blocks_per_cu=0, paired=-1):
+    def set_tuning(self, branches=4, blocks_per_cu=0, paired=-1, queued_stages=1):
         err = C.create_string_buffer(512)
-        t = MixedTuning(branches, blocks_per_cu, paired)
+        t = MixedTuning(branches, blocks_per_cu, paired, queued_stages)
         if lib().pyrsgm_mixed_set_tuning(self._h, C.byref(t), err, 512) != 0:
             raise ValueError(err.value.decode())
 
@@ -660,10 +662,11 @@ class MixedBatch:
         return best.as_dict(), ms.value
 
     def plan(self) -> dict:
-        out = (C.c_longlong * 4)()
+        out = (C.c_longlong * 6)()
         lib().pyrsgm_mixed_plan(self._h, out)
         return {"queued_path_launches": out[0], "path_tasks": out[1], "work_items": out[2],
-                "per_pair_stage_calls": out[3]}
+                "per_pair_stage_calls": out[3], "queued_stage_launches": out[4],
+                "stage_tasks": out[5]}
 
     def device_memory_bytes(self) -> int:
         return lib().pyrsgm_mixed_device_bytes(self._h)

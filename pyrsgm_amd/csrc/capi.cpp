@@ -353,7 +353,7 @@ int computeSingleScale(
     int blockSize, int P1, int P2, int uniquenessRatio, int disp12MaxDiff, int preFilterCap,
     int costMode, int postFlags, int numPaths, int adaptiveP2, float p2Alpha, float p2Beta,
     float p2LowerMult, float p2UpperMult, const Batch& b, hipStream_t stream, int stages,
-    PathSink* sink, const TileConfig* tiles)
+    PathSink* sink, const TileConfig* tiles, StageSink* stageSink)
 {
     if (!workspace || !d_left_gray || !d_right_gray || !geometryOk(width, height, numDisparities) ||
         !paramsOk(blockSize, P1, P2))
@@ -395,6 +395,7 @@ int computeSingleScale(
     c.narrow = narrowVolumes(c);
     c.stages = stages;
     c.sink = sink;
+    c.stageSink = stageSink;
     if (tiles) c.tiles = *tiles;
     c.batch = b;
     c.batch.ws = ws->slice;
@@ -414,7 +415,7 @@ int computeLevel(
     int preFilterCap, int costMode, int postFlags, int numPaths, int refineLevel,
     float minConfidence, int adaptiveP2, float p2Alpha, float p2Beta, float p2LowerMult,
     float p2UpperMult, const Batch& b, hipStream_t stream, int stages, PathSink* sink,
-    const TileConfig* tiles)
+    const TileConfig* tiles, StageSink* stageSink)
 {
     if (!workspace || !d_left_gray || !d_right_gray || !geometryOk(width, height, numDisparities) ||
         (refineLevel && numDisparities > 256) || !paramsOk(blockSize, P1, P2))
@@ -469,6 +470,7 @@ int computeLevel(
     c.narrow = narrowVolumes(c);
     c.stages = stages;
     c.sink = sink;
+    c.stageSink = stageSink;
     if (tiles) c.tiles = *tiles;
     c.batch = b;
     c.batch.ws = ws->slice;

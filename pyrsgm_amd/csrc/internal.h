@@ -80,6 +80,16 @@ struct TileConfig {
     int fused = 0;      // R->L lines finish the winner-take-all: 0 default (off), 1 off, 2 on
 };
 
+// The parent level's results a refinement cost kernel reads (pitches in elements).
+struct RefineIn {
+    const int16_t* pdisp;
+    const float* pconf;
+    int pdpitch, pcpitch, PW, PH;
+    float minConf;
+};
+
+struct StageSink;
+
 // Everything one level needs; filled by the C-ABI entry points.
 struct LevelCall {
     const uint8_t* left; const uint8_t* right; int lstep, rstep;
@@ -95,6 +105,7 @@ struct LevelCall {
     int narrow;      // byte volumes (narrowVolumes)
     int stages;      // kStage* bits
     PathSink* sink;  // non-null: path launches are recorded here instead of issued
+    StageSink* stageSink;  // non-null: pyrDown / prep / cost / post launches are recorded here
     TileConfig tiles;
     Batch batch;
     int32_t lut[256];
@@ -264,7 +275,7 @@ int computeSingleScale(void* workspace, const unsigned char* d_left_gray,
                        int numPaths, int adaptiveP2, float p2Alpha, float p2Beta,
                        float p2LowerMult, float p2UpperMult, const Batch& b, hipStream_t stream,
                        int stages = kStagesAll, PathSink* sink = nullptr,
-                       const TileConfig* tiles = nullptr);
+                       const TileConfig* tiles = nullptr, StageSink* stageSink = nullptr);
 int computeLevel(void* workspace, const unsigned char* d_left_gray,
                  const unsigned char* d_right_gray, const short* d_parent_disp,
                  const float* d_parent_conf, short* d_disp, float* d_disp_float,
@@ -278,12 +289,13 @@ int computeLevel(void* workspace, const unsigned char* d_left_gray,
                  float minConfidence, int adaptiveP2, float p2Alpha, float p2Beta,
                  float p2LowerMult, float p2UpperMult, const Batch& b, hipStream_t stream,
                  int stages = kStagesAll, PathSink* sink = nullptr,
-                 const TileConfig* tiles = nullptr);
+                 const TileConfig* tiles = nullptr, StageSink* stageSink = nullptr);
 // pyrDown of n pairs of (one or two) images; src / dst pair strides in bytes.
+// (sink: the launch is recorded instead of issued.)
 hipError_t launchPyrDownBatch(const uint8_t* srcA, int sstepA, const uint8_t* srcB, int sstepB,
                               size_t sStride, int sw, int sh, uint8_t* dstA, int dstepA,
                               uint8_t* dstB, int dstepB, size_t dStride, int dw, int dh, int n,
-                              hipStream_t stream);
+                              hipStream_t stream, StageSink* sink = nullptr);
 
 // One 2-D copy of a batched copy launch (pitches and row length in bytes).
 struct CopyDesc {
@@ -299,5 +311,89 @@ hipError_t launchCopy2D(const CopyDesc* descs, int count, int maxRowBytes, int m
 hipError_t launchPrep(const LevelCall& c, Workspace& ws, hipStream_t s);
 hipError_t launchCost(const LevelCall& c, Workspace& ws, hipStream_t s);
 hipError_t launchPost(const LevelCall& c, Workspace& ws, hipStream_t s);
+
+// ---- stage launches (pyrDown, descriptor pass, cost volume, post) --------
+// Every stage kernel runs either as a launch of its own or as one task of a
+// queued launch that runs all tasks of one kernel specialisation (the
+// pairs of a mixed batch at one pyramid step): block b of a queued launch
+// belongs to the task whose block range holds it and runs tile
+// (x, y, z) = (local % gx, local / gx % gy, local / (gx gy)) of that task.
+enum StageKind : int {
+    kStPrepCensus = 0, kStPrepBT, kStCensusCost, kStCensusLocal, kStBTRow, kStBoxV, kStBTLocal,
+    kStBTLocalTile, kStPyrDown, kStPost, kStKinds
+};
+// Arguments of every stage kernel (each reads the fields it uses).
+struct StageArgs {
+    const uint8_t* left;  // level images (pyrDown: sources); refinement costs read left only
+    const uint8_t* right;
+    int lstep, rstep;
+    uint8_t* dstA;  // pyrDown outputs
+    uint8_t* dstB;
+    int dstepA, dstepB, sw, sh, nv;
+    size_t sStride, dStride;  // pyrDown pair strides
+    uint2* desc;
+    uint32_t* keys;
+    int32_t* labels;
+    int16_t* C;
+    int16_t* Hs;  // BT row sums
+    RefineIn r;
+    int W, H, D, Dp, minD, maxD, half, cap, geom;
+    const int16_t* disp;  // post
+    const float* conf;
+    int maxDiff, lrcheck, median, strict;
+    int16_t* out16;
+    float* out32;
+    float* outc;
+    size_t s16, s32, sc;
+    Batch batch;
+};
+// Per-call fields a queued task takes from its pair's I/O entry: the input
+// images (left, right and their pitches) and the outputs of level 0.
+constexpr int kIoIn = 1, kIoOut = 2;
+struct StageLaunch {
+    int kind, variant;  // kernel and template variant
+    dim3 grid;          // per-launch grid (one task: grid.z counts this pair's slices)
+    size_t lds;         // dynamic LDS bytes
+    StageArgs a;
+    int io;             // kIoIn / kIoOut fields taken from the caller's I/O when queued
+};
+// Stage launches recorded instead of issued (LevelCall::stageSink).
+struct StageSink {
+    std::vector<StageLaunch> launches;
+};
+hipError_t launchStage(const StageLaunch& l, hipStream_t s);
+// The caller's buffers of one pair, as the queued tasks read them; outc is
+// already routed (float disparities without a confidence output write the
+// confidence to the pair's scratch).
+struct StageIo {
+    const uint8_t* left;
+    const uint8_t* right;
+    int16_t* out16;
+    float* out32;
+    float* outc;
+    size_t lstep, rstep, s16, s32, sc;
+};
+struct StageTask {
+    StageArgs a;
+    int gx, gy;  // tile grid of the task (x, y; z is the rest)
+    int io, pair;
+};
+struct StageQueue {
+    const StageTask* tasks;  // device [nTasks]
+    const int* start;        // device [nTasks + 1] first block of each task
+    const StageIo* io;       // device [pairs]
+    int nTasks;
+};
+hipError_t launchStageQueued(int kind, int variant, const StageQueue& q, int blocks, size_t lds,
+                             hipStream_t s);
+// Writes n entries of the device I/O table (table[idx[i]] = io[i]) in one launch.
+constexpr int kMaxIoPatch = 32;
+struct IoPatch {
+    StageIo* table;
+    int n;
+    int idx[kMaxIoPatch];
+    StageIo io[kMaxIoPatch];
+};
+hipError_t launchIoPatch(const IoPatch& p, hipStream_t s);
 
 } // namespace pyr

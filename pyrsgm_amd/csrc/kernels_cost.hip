@@ -10,11 +10,14 @@
 // Volumes are int16 [y][x][Dp], labels innermost, stored as SHRT_MAX - cost;
 // every thread produces 8 consecutive labels of one pixel and writes them with
 // one 16-byte store.
+#include <string.h>
+
 #include <climits>
 #include <type_traits>
 
 #include "device_common.h"
 #include "internal.h"
+#include "stage_impl.h"
 
 namespace pyr {
 namespace {
@@ -72,13 +75,12 @@ __device__ __forceinline__ int byteOf(uint32_t lo, uint32_t hi, int b)
     return (int)(((b < 4 ? lo : hi) >> (8 * (b & 3))) & 0xFFu);
 }
 
-__global__ __launch_bounds__(256) void kPrepCensus(const uint8_t* left, int lstep,
-                                                   const uint8_t* right, int rstep, uint32_t* cen,
-                                                   uint32_t* keys, int W, int H, Batch b)
+__device__ __forceinline__ void kPrepCensusBody(Tile3 tl, const uint8_t* left, int lstep,
+                                                const uint8_t* right, int rstep, uint32_t* cen,
+                                                uint32_t* keys, int W, int H, Batch b)
 {
     constexpr int TW = 68, TS = 72, TH = kCenRows + 4, TILE = TW * TH, NL = (TILE + 255) / 256;
     __shared__ __attribute__((aligned(16))) uint8_t s_t[TH][TS];
-    const Tile3 tl = xcdTile();
     const int z = tl.z & 1;          // view
     const size_t pr = tl.z >> 1;     // pair
     const uint8_t* img = z ? pairAt(right, pr, b.right) : pairAt(left, pr, b.left);
@@ -147,11 +149,10 @@ __global__ __launch_bounds__(256) void kPrepCensus(const uint8_t* left, int lste
 }
 
 // BT: the packed intervals of a pixel (x: gradient channel, y: intensity).
-__global__ __launch_bounds__(256) void kPrepBT(const uint8_t* left, int lstep, const uint8_t* right,
-                                               int rstep, uint2* desc, uint32_t* keys, int W, int H,
-                                               int cap, Batch b)
+__device__ __forceinline__ void kPrepBTBody(Tile3 tl, const uint8_t* left, int lstep,
+                                            const uint8_t* right, int rstep, uint2* desc,
+                                            uint32_t* keys, int W, int H, int cap, Batch b)
 {
-    const Tile3 tl = xcdTile();
     const int x = tl.x * 64 + (threadIdx.x & 63);
     const int y = tl.y * 4 + (threadIdx.x >> 6);
     const int z = tl.z & 1;          // view
@@ -216,11 +217,10 @@ __device__ __forceinline__ void storeCensus8(int16_t* C, size_t at, const int (&
 // SHRT_MAX: that is exactly the reference's labelValid = geometry && C < MAX
 // (sgm_kernels.cu:1135-1143), so the path kernels test "C < SHRT_MAX" alone.
 template <bool NARROW>
-__global__ __launch_bounds__(256) void kCensusCost(const uint32_t* cen, int16_t* C, int W, int H,
-                                                   int D, int Dp, int minD, int maxD, int geomMask,
-                                                   size_t wsStride)
+__device__ __forceinline__ void kCensusCostBody(Tile3 tl, const uint32_t* cen, int16_t* C, int W,
+                                                int H, int D, int Dp, int minD, int maxD,
+                                                int geomMask, size_t wsStride)
 {
-    const Tile3 tl = xcdTile();
     extern __shared__ uint32_t s_r[];
     cen = pairAt(cen, tl.z, wsStride);
     C = pairAt(C, tl.z, wsStride);
@@ -286,13 +286,6 @@ __global__ __launch_bounds__(256) void kCensusCost(const uint32_t* cen, int16_t*
 }
 
 // ---- refinement labels (sgm_kernels.cu:955-1000) ----------------------------
-struct RefineIn {
-    const int16_t* pdisp;
-    const float* pconf;
-    int pdpitch, pcpitch, PW, PH;
-    float minConf;
-};
-
 // The window of pixel x from its parent's raw disparity and confidence.
 __device__ __forceinline__ void refineFrom(const RefineIn& r, int raw, float cf, int x, int D,
                                            int minD, int maxD, int& base, int& valid)
@@ -346,12 +339,11 @@ __device__ __forceinline__ int32_t labelWord(int base, int valid, uint8_t im)
 // (staged words, parent disparity and confidence, left census word,
 // intensity, for all its rows) is issued before the first wait.
 template <bool STAGED, bool NARROW, int ROWS>
-__global__ __launch_bounds__(256) void kCensusLocal(const uint32_t* cen, RefineIn r, int32_t* labels,
-                                                    const uint8_t* img, int istep, int16_t* C,
-                                                    int W, int H, int D, int Dp, int minD, int maxD,
-                                                    Batch b)
+__device__ __forceinline__ void kCensusLocalBody(Tile3 tl, const uint32_t* cen, RefineIn r,
+                                                 int32_t* labels, const uint8_t* img, int istep,
+                                                 int16_t* C, int W, int H, int D, int Dp, int minD,
+                                                 int maxD, Batch b)
 {
-    const Tile3 tl = xcdTile();
     constexpr int NPRE = 2;  // staged words per thread and row loaded ahead
     extern __shared__ uint32_t s_r[];
     const size_t pr = tl.z;
@@ -502,19 +494,19 @@ __device__ __forceinline__ int btCostPacked(const BTPacked& l, const BTPacked& r
 constexpr int kSegX = 64;
 constexpr int kChunkD = 64;
 
-__global__ __launch_bounds__(256) void kBTRow(const uint2* desc, int16_t* Hs, int W, int H,
-                                              int Dp, int minD, int half, size_t wsStride)
+__device__ __forceinline__ void kBTRowBody(Tile3 tl, const uint2* desc, int16_t* Hs, int W, int H,
+                                           int Dp, int minD, int half, size_t wsStride)
 {
     extern __shared__ __attribute__((aligned(16))) uint8_t s_bt[];
     const int dc = imin(Dp, kChunkD);      // a power of two
     const int dsh = 31 - __clz(dc);
     const int nzc = Dp / dc;  // label chunks per pair
-    const size_t pr = blockIdx.z / nzc;
+    const size_t pr = tl.z / nzc;
     desc = pairAt(desc, pr, wsStride);
     Hs = pairAt(Hs, pr, wsStride);
-    const int y = blockIdx.y;
-    const int x0 = blockIdx.x * kSegX;
-    const int dc0 = (blockIdx.z - (int)pr * nzc) * dc;
+    const int y = tl.y;
+    const int x0 = tl.x * kSegX;
+    const int dc0 = (tl.z - (int)pr * nzc) * dc;
     const int ext = kSegX + 2 * half;
     const uint2* L = desc + (size_t)y * W;
     const uint2* R = desc + (size_t)W * H + (size_t)y * W;
@@ -560,13 +552,13 @@ __global__ __launch_bounds__(256) void kBTRow(const uint2* desc, int16_t* Hs, in
         store8(Hs + ((size_t)y * W + x) * Dp + dc0 + dl0, v);
     }
 }
-__global__ __launch_bounds__(256) void kBoxV(const int16_t* Hs, int16_t* C, int W, int H, int D,
-                                             int Dp, int half, int minD, int maxD, int geomMask,
-                                             size_t wsStride)
+__device__ __forceinline__ void kBoxVBody(Tile3 tl, const int16_t* Hs, int16_t* C, int W, int H,
+                                          int D, int Dp, int half, int minD, int maxD, int geomMask,
+                                          size_t wsStride)
 {
-    Hs = pairAt(Hs, blockIdx.y, wsStride);
-    C = pairAt(C, blockIdx.y, wsStride);
-    const size_t oct = (size_t)blockIdx.x * 256 + threadIdx.x;
+    Hs = pairAt(Hs, tl.y, wsStride);
+    C = pairAt(C, tl.y, wsStride);
+    const size_t oct = (size_t)tl.x * 256 + threadIdx.x;
     const size_t nOct = (size_t)W * H * (Dp >> 3);
     if (oct >= nOct) return;
     const size_t p = oct / (Dp >> 3);
@@ -657,12 +649,12 @@ __device__ __forceinline__ void btWindowInterior(const uint2* Ld, const uint2* R
 }
 
 template <bool CLAMP>
-__global__ __launch_bounds__(256) void kBTLocal(const uint2* desc, RefineIn r, int32_t* labels,
-                                                const uint8_t* img, int istep, int16_t* C, int W,
-                                                int H, int D, int Dp, int minD, int maxD, int half,
-                                                Batch b)
+__device__ __forceinline__ void kBTLocalBody(Tile3 tl, const uint2* desc, RefineIn r,
+                                             int32_t* labels, const uint8_t* img, int istep,
+                                             int16_t* C, int W, int H, int D, int Dp, int minD,
+                                             int maxD, int half, Batch b)
 {
-    const size_t pr = blockIdx.z;
+    const size_t pr = tl.z;
     desc = pairAt(desc, pr, b.ws);
     labels = pairAt(labels, pr, b.ws);
     C = pairAt(C, pr, b.ws);
@@ -671,10 +663,10 @@ __global__ __launch_bounds__(256) void kBTLocal(const uint2* desc, RefineIn r, i
     r.pconf = pairAt(r.pconf, pr, b.pconf);
     const int tpp = Dp >> 3;
     const int npx = 256 / tpp;
-    const int y = blockIdx.y;
+    const int y = tl.y;
     const int px = threadIdx.x / tpp;
     const int d0 = (threadIdx.x - px * tpp) * 8;
-    const int x = blockIdx.x * npx + px;
+    const int x = tl.x * npx + px;
     if (x >= W) return;
     int base, valid;
     refineLabel(r, x, y, D, minD, maxD, base, valid);
@@ -736,11 +728,11 @@ constexpr int kTileRows = 4;
 constexpr int kTileR = 48;
 
 template <int HALF>
-__global__ __launch_bounds__(256) void kBTLocalTile(const uint2* desc, RefineIn r, int32_t* labels,
-                                                    const uint8_t* img, int istep, int16_t* C,
-                                                    int W, int H, int minD, int maxD, Batch b)
+__device__ __forceinline__ void kBTLocalTileBody(Tile3 tl, const uint2* desc, RefineIn r,
+                                                 int32_t* labels, const uint8_t* img, int istep,
+                                                 int16_t* C, int W, int H, int minD, int maxD,
+                                                 Batch b)
 {
-    const Tile3 tl = xcdTile();
     typedef uint16_t u2 __attribute__((ext_vector_type(2)));
     constexpr int NC = kTilePx + 2 * HALF;       // window columns of the block
     constexpr int NY = 2 * HALF + 1;             // window rows of one output row
@@ -929,12 +921,11 @@ __device__ __forceinline__ int reflect101(int i, int n)
 // registers with every load in flight at once, then staged in LDS.
 constexpr int kPyrRows = 8;
 
-__global__ __launch_bounds__(256) void kPyrDown(const uint8_t* srcA, int sstepA, const uint8_t* srcB,
-                                                int sstepB, size_t sStride, int sw, int sh,
-                                                uint8_t* dstA, int dstepA, uint8_t* dstB,
-                                                int dstepB, size_t dStride, int dw, int dh, int nv)
+__device__ __forceinline__ void kPyrDownBody(Tile3 tl, const uint8_t* srcA, int sstepA,
+                                             const uint8_t* srcB, int sstepB, size_t sStride, int sw,
+                                             int sh, uint8_t* dstA, int dstepA, uint8_t* dstB,
+                                             int dstepB, size_t dStride, int dw, int dh, int nv)
 {
-    const Tile3 tl = xcdTile();
     constexpr int TW = 131, TH = 2 * kPyrRows + 3, TILE = TW * TH, NL = (TILE + 255) / 256;
     __shared__ uint8_t s_t[TH][TW + 1];
     const size_t pr = tl.z / nv;              // pair
@@ -978,19 +969,222 @@ __global__ __launch_bounds__(256) void kPyrDown(const uint8_t* srcA, int sstepA,
     }
 }
 
+// ---- per-launch and queued kernels ----------------------------------------
+// Variant codes: kStCensusCost 0/1 = int16 / byte volumes; kStCensusLocal
+// staged | narrow << 1 | (rows 2/4/8 -> 0/1/2) << 2; kStBTLocal 0/1 = clamp
+// off / on; kStBTLocalTile = the window's half size.
+constexpr int censusLocalVariant(bool staged, bool narrow, int rows)
+{
+    return (staged ? 1 : 0) | (narrow ? 2 : 0) | ((rows == 2 ? 0 : (rows == 8 ? 2 : 1)) << 2);
+}
+constexpr int censusLocalRows(int v) { return (v >> 2) == 0 ? 2 : ((v >> 2) == 2 ? 8 : 4); }
+
+template <int KIND, int V>
+__device__ __forceinline__ void stageBody(const StageArgs& a, Tile3 tl)
+{
+    const uint32_t* cen = reinterpret_cast<const uint32_t*>(a.desc);
+    if constexpr (KIND == kStPrepCensus)
+        kPrepCensusBody(tl, a.left, a.lstep, a.right, a.rstep, reinterpret_cast<uint32_t*>(a.desc),
+                        a.keys, a.W, a.H, a.batch);
+    else if constexpr (KIND == kStPrepBT)
+        kPrepBTBody(tl, a.left, a.lstep, a.right, a.rstep, a.desc, a.keys, a.W, a.H, a.cap, a.batch);
+    else if constexpr (KIND == kStCensusCost)
+        kCensusCostBody<V != 0>(tl, cen, a.C, a.W, a.H, a.D, a.Dp, a.minD, a.maxD, a.geom, a.batch.ws);
+    else if constexpr (KIND == kStCensusLocal)
+        kCensusLocalBody<(V & 1) != 0, (V & 2) != 0, censusLocalRows(V)>(
+            tl, cen, a.r, a.labels, a.left, a.lstep, a.C, a.W, a.H, a.D, a.Dp, a.minD, a.maxD, a.batch);
+    else if constexpr (KIND == kStBTRow)
+        kBTRowBody(tl, a.desc, a.Hs, a.W, a.H, a.Dp, a.minD, a.half, a.batch.ws);
+    else if constexpr (KIND == kStBoxV)
+        kBoxVBody(tl, a.Hs, a.C, a.W, a.H, a.D, a.Dp, a.half, a.minD, a.maxD, a.geom, a.batch.ws);
+    else if constexpr (KIND == kStBTLocal)
+        kBTLocalBody<V != 0>(tl, a.desc, a.r, a.labels, a.left, a.lstep, a.C, a.W, a.H, a.D, a.Dp,
+                             a.minD, a.maxD, a.half, a.batch);
+    else if constexpr (KIND == kStBTLocalTile)
+        kBTLocalTileBody<V>(tl, a.desc, a.r, a.labels, a.left, a.lstep, a.C, a.W, a.H, a.minD,
+                            a.maxD, a.batch);
+    else if constexpr (KIND == kStPyrDown)
+        kPyrDownBody(tl, a.left, a.lstep, a.right, a.rstep, a.sStride, a.sw, a.sh, a.dstA, a.dstepA,
+                     a.dstB, a.dstepB, a.dStride, a.W, a.H, a.nv);
+}
+
+// Per-launch kernels: tiles XCD-contiguous, except the BT row, box and plain
+// local kernels, which keep the hardware order.
+__global__ __launch_bounds__(256) void kPrepCensus(StageArgs a) { stageBody<kStPrepCensus, 0>(a, xcdTile()); }
+__global__ __launch_bounds__(256) void kPrepBT(StageArgs a) { stageBody<kStPrepBT, 0>(a, xcdTile()); }
+template <bool NARROW>
+__global__ __launch_bounds__(256) void kCensusCost(StageArgs a)
+{
+    stageBody<kStCensusCost, NARROW ? 1 : 0>(a, xcdTile());
+}
+template <bool STAGED, bool NARROW, int ROWS>
+__global__ __launch_bounds__(256) void kCensusLocal(StageArgs a)
+{
+    stageBody<kStCensusLocal, censusLocalVariant(STAGED, NARROW, ROWS)>(a, xcdTile());
+}
+__global__ __launch_bounds__(256) void kBTRow(StageArgs a) { stageBody<kStBTRow, 0>(a, hwTile()); }
+__global__ __launch_bounds__(256) void kBoxV(StageArgs a) { stageBody<kStBoxV, 0>(a, hwTile()); }
+template <bool CLAMP>
+__global__ __launch_bounds__(256) void kBTLocal(StageArgs a)
+{
+    stageBody<kStBTLocal, CLAMP ? 1 : 0>(a, hwTile());
+}
+template <int HALF>
+__global__ __launch_bounds__(256) void kBTLocalTile(StageArgs a)
+{
+    stageBody<kStBTLocalTile, HALF>(a, xcdTile());
+}
+__global__ __launch_bounds__(256) void kPyrDown(StageArgs a) { stageBody<kStPyrDown, 0>(a, xcdTile()); }
+
+template <int KIND, int V>
+__global__ __launch_bounds__(256) void kStageQ(StageQueue q)
+{
+    runQueuedStage(q, [](const StageArgs& a, Tile3 tl) { stageBody<KIND, V>(a, tl); });
+}
+
+template <bool STAGED, bool NARROW, int ROWS>
+hipError_t launchCensusLocal(const StageLaunch& l, hipStream_t s)
+{
+    kCensusLocal<STAGED, NARROW, ROWS><<<l.grid, 256, l.lds, s>>>(l.a);
+    return hipGetLastError();
+}
+
+template <int V>
+hipError_t launchQ(int kind, const StageQueue& q, int blocks, size_t lds, hipStream_t s)
+{
+    switch (kind) {
+    case kStCensusCost: kStageQ<kStCensusCost, V & 1><<<blocks, 256, lds, s>>>(q); break;
+    case kStCensusLocal: kStageQ<kStCensusLocal, V><<<blocks, 256, lds, s>>>(q); break;
+    case kStBTLocal: kStageQ<kStBTLocal, V & 1><<<blocks, 256, lds, s>>>(q); break;
+    case kStBTLocalTile: kStageQ<kStBTLocalTile, V & 3><<<blocks, 256, lds, s>>>(q); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// Output of a stage launcher: recorded (LevelCall::stageSink) or launched.
+hipError_t emit(const LevelCall* c, const StageLaunch& l, hipStream_t s)
+{
+    if (c && c->stageSink) {
+        c->stageSink->launches.push_back(l);
+        return hipSuccess;
+    }
+    return launchStage(l, s);
+}
+
+StageLaunch stageOf(int kind, int variant, dim3 grid, size_t lds)
+{
+    StageLaunch l;
+    memset(&l, 0, sizeof(l));
+    l.kind = kind;
+    l.variant = variant;
+    l.grid = grid;
+    l.lds = lds;
+    return l;
+}
+
 } // namespace
+
+hipError_t launchPostStage(const StageLaunch& l, hipStream_t s);
+hipError_t launchPostQueued(const StageQueue& q, int blocks, hipStream_t s);
+
+hipError_t launchStage(const StageLaunch& l, hipStream_t s)
+{
+    const int v = l.variant;
+    switch (l.kind) {
+    case kStPrepCensus: kPrepCensus<<<l.grid, 256, l.lds, s>>>(l.a); break;
+    case kStPrepBT: kPrepBT<<<l.grid, 256, l.lds, s>>>(l.a); break;
+    case kStCensusCost:
+        if (v) kCensusCost<true><<<l.grid, 256, l.lds, s>>>(l.a);
+        else kCensusCost<false><<<l.grid, 256, l.lds, s>>>(l.a);
+        break;
+    case kStCensusLocal:
+        switch (v) {
+        case censusLocalVariant(true, true, 2): return launchCensusLocal<true, true, 2>(l, s);
+        case censusLocalVariant(true, true, 4): return launchCensusLocal<true, true, 4>(l, s);
+        case censusLocalVariant(true, true, 8): return launchCensusLocal<true, true, 8>(l, s);
+        case censusLocalVariant(true, false, 2): return launchCensusLocal<true, false, 2>(l, s);
+        case censusLocalVariant(true, false, 4): return launchCensusLocal<true, false, 4>(l, s);
+        case censusLocalVariant(true, false, 8): return launchCensusLocal<true, false, 8>(l, s);
+        case censusLocalVariant(false, true, 2): return launchCensusLocal<false, true, 2>(l, s);
+        case censusLocalVariant(false, true, 4): return launchCensusLocal<false, true, 4>(l, s);
+        case censusLocalVariant(false, true, 8): return launchCensusLocal<false, true, 8>(l, s);
+        case censusLocalVariant(false, false, 2): return launchCensusLocal<false, false, 2>(l, s);
+        case censusLocalVariant(false, false, 4): return launchCensusLocal<false, false, 4>(l, s);
+        case censusLocalVariant(false, false, 8): return launchCensusLocal<false, false, 8>(l, s);
+        default: return hipErrorInvalidValue;
+        }
+    case kStBTRow: kBTRow<<<l.grid, 256, l.lds, s>>>(l.a); break;
+    case kStBoxV: kBoxV<<<l.grid, 256, l.lds, s>>>(l.a); break;
+    case kStBTLocal:
+        if (v) kBTLocal<true><<<l.grid, 256, l.lds, s>>>(l.a);
+        else kBTLocal<false><<<l.grid, 256, l.lds, s>>>(l.a);
+        break;
+    case kStBTLocalTile:
+        switch (v) {
+        case 0: kBTLocalTile<0><<<l.grid, 256, l.lds, s>>>(l.a); break;
+        case 1: kBTLocalTile<1><<<l.grid, 256, l.lds, s>>>(l.a); break;
+        case 2: kBTLocalTile<2><<<l.grid, 256, l.lds, s>>>(l.a); break;
+        case 3: kBTLocalTile<3><<<l.grid, 256, l.lds, s>>>(l.a); break;
+        default: return hipErrorInvalidValue;
+        }
+        break;
+    case kStPyrDown: kPyrDown<<<l.grid, 256, l.lds, s>>>(l.a); break;
+    case kStPost: return launchPostStage(l, s);
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launchStageQueued(int kind, int variant, const StageQueue& q, int blocks, size_t lds,
+                             hipStream_t s)
+{
+    if (blocks <= 0) return hipSuccess;
+    switch (kind) {
+    case kStPrepCensus: kStageQ<kStPrepCensus, 0><<<blocks, 256, lds, s>>>(q); break;
+    case kStPrepBT: kStageQ<kStPrepBT, 0><<<blocks, 256, lds, s>>>(q); break;
+    case kStBTRow: kStageQ<kStBTRow, 0><<<blocks, 256, lds, s>>>(q); break;
+    case kStBoxV: kStageQ<kStBoxV, 0><<<blocks, 256, lds, s>>>(q); break;
+    case kStPyrDown: kStageQ<kStPyrDown, 0><<<blocks, 256, lds, s>>>(q); break;
+    case kStPost: return launchPostQueued(q, blocks, s);
+    default:
+        switch (variant) {
+        case 0: return launchQ<0>(kind, q, blocks, lds, s);
+        case 1: return launchQ<1>(kind, q, blocks, lds, s);
+        case 2: return launchQ<2>(kind, q, blocks, lds, s);
+        case 3: return launchQ<3>(kind, q, blocks, lds, s);
+        case 4: return launchQ<4>(kind, q, blocks, lds, s);
+        case 5: return launchQ<5>(kind, q, blocks, lds, s);
+        case 6: return launchQ<6>(kind, q, blocks, lds, s);
+        case 7: return launchQ<7>(kind, q, blocks, lds, s);
+        case 8: return launchQ<8>(kind, q, blocks, lds, s);
+        case 9: return launchQ<9>(kind, q, blocks, lds, s);
+        case 10: return launchQ<10>(kind, q, blocks, lds, s);
+        case 11: return launchQ<11>(kind, q, blocks, lds, s);
+        default: return hipErrorInvalidValue;
+        }
+    }
+    return hipGetLastError();
+}
 
 hipError_t launchPrep(const LevelCall& c, Workspace& ws, hipStream_t s)
 {
-    if (c.cost == 1)
-        kPrepCensus<<<dim3((c.W + 63) / 64, (c.H + kCenRows - 1) / kCenRows, 2 * c.batch.n), 256,
-                      0, s>>>(c.left, c.lstep, c.right, c.rstep,
-                                         reinterpret_cast<uint32_t*>(ws.desc), ws.keys, c.W, c.H,
-                                         c.batch);
-    else
-        kPrepBT<<<dim3((c.W + 63) / 64, (c.H + 3) / 4, 2 * c.batch.n), 256, 0, s>>>(c.left, c.lstep, c.right, c.rstep, ws.desc, ws.keys, c.W,
-                                     c.H, c.cap, c.batch);
-    return hipGetLastError();
+    const bool census = c.cost == 1;
+    StageLaunch l = census
+        ? stageOf(kStPrepCensus, 0, dim3((c.W + 63) / 64, (c.H + kCenRows - 1) / kCenRows, 2 * c.batch.n), 0)
+        : stageOf(kStPrepBT, 0, dim3((c.W + 63) / 64, (c.H + 3) / 4, 2 * c.batch.n), 0);
+    StageArgs& a = l.a;
+    a.left = c.left;
+    a.lstep = c.lstep;
+    a.right = c.right;
+    a.rstep = c.rstep;
+    a.desc = ws.desc;
+    a.keys = ws.keys;
+    a.W = c.W;
+    a.H = c.H;
+    a.cap = c.cap;
+    a.batch = c.batch;
+    return emit(&c, l, s);
 }
 
 // Right census words per row a refinement block may stage in LDS (4 rows: 32 KB).
@@ -1001,86 +1195,85 @@ hipError_t launchCost(const LevelCall& c, Workspace& ws, hipStream_t s)
     const int tpp = c.Dp / 8;
     const int npx = 256 / tpp;
     const int half = c.block / 2;
+    StageArgs a;
+    memset(&a, 0, sizeof(a));
+    a.desc = ws.desc;
+    a.labels = ws.labels;
+    a.C = ws.C;
+    a.Hs = ws.Alr;
+    a.left = c.left;
+    a.lstep = c.lstep;
+    a.W = c.W;
+    a.H = c.H;
+    a.D = c.D;
+    a.Dp = c.Dp;
+    a.minD = c.minD;
+    a.maxD = c.maxD;
+    a.half = half;
+    a.geom = c.mode == kCoarse;
+    a.batch = c.batch;
+    auto go = [&](int kind, int variant, dim3 grid, size_t lds) {
+        StageLaunch l = stageOf(kind, variant, grid, lds);
+        l.a = a;
+        return emit(&c, l, s);
+    };
     if (c.mode == kRefine) {
-        RefineIn r{c.pdisp, c.pconf, c.pdpitch, c.pcpitch, c.PW, c.PH, c.minConf};
-        dim3 grid((c.W + npx - 1) / npx, c.H, c.batch.n);
-        const uint32_t* cen = reinterpret_cast<const uint32_t*>(ws.desc);
+        a.r = RefineIn{c.pdisp, c.pconf, c.pdpitch, c.pcpitch, c.PW, c.PH, c.minConf};
+        const dim3 grid((c.W + npx - 1) / npx, c.H, c.batch.n);
         const long span = (long)npx + c.maxD - c.minD;  // staged right columns per row
         if (c.cost == 1) {
             const int rows = c.tiles.localRows ? c.tiles.localRows : 4;
-            auto census = [&](auto rowsC) {
-                constexpr int ROWS = decltype(rowsC)::value;
-                const dim3 g(grid.x, (c.H + ROWS - 1) / ROWS, grid.z);
-                const size_t lds = (size_t)span * 4 * ROWS;
-                if (span <= kMaxStaged && c.narrow)
-                    kCensusLocal<true, true, ROWS><<<g, 256, lds, s>>>(
-                        cen, r, ws.labels, c.left, c.lstep, ws.C, c.W, c.H, c.D, c.Dp, c.minD, c.maxD, c.batch);
-                else if (span <= kMaxStaged)
-                    kCensusLocal<true, false, ROWS><<<g, 256, lds, s>>>(
-                        cen, r, ws.labels, c.left, c.lstep, ws.C, c.W, c.H, c.D, c.Dp, c.minD, c.maxD, c.batch);
-                else if (c.narrow)
-                    kCensusLocal<false, true, ROWS><<<g, 256, 0, s>>>(
-                        cen, r, ws.labels, c.left, c.lstep, ws.C, c.W, c.H, c.D, c.Dp, c.minD, c.maxD, c.batch);
-                else
-                    kCensusLocal<false, false, ROWS><<<g, 256, 0, s>>>(
-                        cen, r, ws.labels, c.left, c.lstep, ws.C, c.W, c.H, c.D, c.Dp, c.minD, c.maxD, c.batch);
-            };
-            if (rows == 2) census(std::integral_constant<int, 2>{});
-            else if (rows == 8) census(std::integral_constant<int, 8>{});
-            else census(std::integral_constant<int, 4>{});
+            const bool staged = span <= kMaxStaged;
+            const dim3 g(grid.x, (c.H + rows - 1) / rows, grid.z);
+            return go(kStCensusLocal, censusLocalVariant(staged, c.narrow != 0, rows), g,
+                      staged ? (size_t)span * 4 * rows : 0);
         }
-        else if (((c.cap > 15 ? c.cap : 15) | 1) <= 95 && c.Dp == 16 && c.D == 16 && half <= 3) {
-            const dim3 tg((c.W + kTilePx - 1) / kTilePx, (c.H + kTileRows - 1) / kTileRows, c.batch.n);
-            switch (half) {
-            case 0: kBTLocalTile<0><<<tg, 256, 0, s>>>(ws.desc, r, ws.labels, c.left, c.lstep, ws.C, c.W, c.H, c.minD, c.maxD, c.batch); break;
-            case 1: kBTLocalTile<1><<<tg, 256, 0, s>>>(ws.desc, r, ws.labels, c.left, c.lstep, ws.C, c.W, c.H, c.minD, c.maxD, c.batch); break;
-            case 2: kBTLocalTile<2><<<tg, 256, 0, s>>>(ws.desc, r, ws.labels, c.left, c.lstep, ws.C, c.W, c.H, c.minD, c.maxD, c.batch); break;
-            default: kBTLocalTile<3><<<tg, 256, 0, s>>>(ws.desc, r, ws.labels, c.left, c.lstep, ws.C, c.W, c.H, c.minD, c.maxD, c.batch); break;
-            }
-        } else if (((c.cap > 15 ? c.cap : 15) | 1) > 95)  // a cost may reach 255
-            kBTLocal<true><<<grid, 256, 0, s>>>(ws.desc, r, ws.labels, c.left, c.lstep, ws.C, c.W,
-                                                c.H, c.D, c.Dp, c.minD, c.maxD, half, c.batch);
-        else
-            kBTLocal<false><<<grid, 256, 0, s>>>(ws.desc, r, ws.labels, c.left, c.lstep, ws.C, c.W,
-                                                 c.H, c.D, c.Dp, c.minD, c.maxD, half, c.batch);
-        return hipGetLastError();
+        if (((c.cap > 15 ? c.cap : 15) | 1) <= 95 && c.Dp == 16 && c.D == 16 && half <= 3)
+            return go(kStBTLocalTile, half,
+                      dim3((c.W + kTilePx - 1) / kTilePx, (c.H + kTileRows - 1) / kTileRows, c.batch.n), 0);
+        // Clamped sums when a cost may reach 255.
+        return go(kStBTLocal, ((c.cap > 15 ? c.cap : 15) | 1) > 95 ? 1 : 0, grid, 0);
     }
-    if (c.cost == 1) {
-        dim3 grid((c.W + npx - 1) / npx, c.H, c.batch.n);
-        const size_t lds = (size_t)(npx + c.Dp - 1) * sizeof(uint32_t);
-        const uint32_t* cen = reinterpret_cast<const uint32_t*>(ws.desc);
-        if (c.narrow)
-            kCensusCost<true><<<grid, 256, lds, s>>>(cen, ws.C, c.W, c.H, c.D, c.Dp, c.minD,
-                                                     c.maxD, c.mode == kCoarse, c.batch.ws);
-        else
-            kCensusCost<false><<<grid, 256, lds, s>>>(cen, ws.C, c.W, c.H, c.D, c.Dp, c.minD,
-                                                      c.maxD, c.mode == kCoarse, c.batch.ws);
-        return hipGetLastError();
-    }
+    if (c.cost == 1)
+        return go(kStCensusCost, c.narrow ? 1 : 0, dim3((c.W + npx - 1) / npx, c.H, c.batch.n),
+                  (size_t)(npx + c.Dp - 1) * sizeof(uint32_t));
     const int dc = c.Dp < kChunkD ? c.Dp : kChunkD;
-    dim3 gridH((c.W + kSegX - 1) / kSegX, c.H, (c.Dp / dc) * c.batch.n);
     const int ext = kSegX + 2 * half;
     const size_t lds = (size_t)((ext * dc + 15) & ~15) + (size_t)(2 * ext + dc) * sizeof(BTPacked);
-    kBTRow<<<gridH, 256, lds, s>>>(ws.desc, ws.Alr, c.W, c.H, c.Dp, c.minD, half, c.batch.ws);
-    hipError_t e = hipGetLastError();
+    hipError_t e = go(kStBTRow, 0, dim3((c.W + kSegX - 1) / kSegX, c.H, (c.Dp / dc) * c.batch.n), lds);
     if (e != hipSuccess) return e;
     const size_t nOct = (size_t)c.W * c.H * (c.Dp / 8);
-    dim3 gridV((unsigned)((nOct + 255) / 256), c.batch.n);
-    kBoxV<<<gridV, 256, 0, s>>>(ws.Alr, ws.C, c.W, c.H, c.D, c.Dp, half, c.minD, c.maxD,
-                                c.mode == kCoarse, c.batch.ws);
-    return hipGetLastError();
+    return go(kStBoxV, 0, dim3((unsigned)((nOct + 255) / 256), c.batch.n), 0);
 }
 
 hipError_t launchPyrDownBatch(const uint8_t* srcA, int sstepA, const uint8_t* srcB, int sstepB,
                               size_t sStride, int sw, int sh, uint8_t* dstA, int dstepA,
                               uint8_t* dstB, int dstepB, size_t dStride, int dw, int dh, int n,
-                              hipStream_t stream)
+                              hipStream_t stream, StageSink* sink)
 {
     const int nv = srcB ? 2 : 1;
-    dim3 grid((dw + 63) / 64, (dh + kPyrRows - 1) / kPyrRows, nv * n);
-    kPyrDown<<<grid, 256, 0, stream>>>(srcA, sstepA, srcB, sstepB, sStride, sw, sh, dstA, dstepA,
-                                       dstB, dstepB, dStride, dw, dh, nv);
-    return hipGetLastError();
+    StageLaunch l = stageOf(kStPyrDown, 0, dim3((dw + 63) / 64, (dh + kPyrRows - 1) / kPyrRows, nv * n), 0);
+    StageArgs& a = l.a;
+    a.left = srcA;
+    a.lstep = sstepA;
+    a.right = srcB;
+    a.rstep = sstepB;
+    a.sStride = sStride;
+    a.sw = sw;
+    a.sh = sh;
+    a.dstA = dstA;
+    a.dstepA = dstepA;
+    a.dstB = dstB;
+    a.dstepB = dstepB;
+    a.dStride = dStride;
+    a.W = dw;
+    a.H = dh;
+    a.nv = nv;
+    if (sink) {
+        sink->launches.push_back(l);
+        return hipSuccess;
+    }
+    return launchStage(l, stream);
 }
 
 hipError_t launchPyrDown2(const uint8_t* srcA, int sstepA, const uint8_t* srcB, int sstepB,
@@ -1088,7 +1281,7 @@ hipError_t launchPyrDown2(const uint8_t* srcA, int sstepA, const uint8_t* srcB, 
                           int dw, int dh, hipStream_t stream)
 {
     return launchPyrDownBatch(srcA, sstepA, srcB, sstepB, 0, sw, sh, dstA, dstepA, dstB, dstepB,
-                              0, dw, dh, 1, stream);
+                              0, dw, dh, 1, stream, nullptr);
 }
 
 hipError_t launchPyrDown(const uint8_t* src, int sstep, int sw, int sh, uint8_t* dst, int dstep,

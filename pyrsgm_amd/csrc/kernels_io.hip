@@ -38,7 +38,22 @@ __global__ __launch_bounds__(64) void kImgPatch(ImgPatch p)
     t->istep = p.istep[i];
 }
 
+__global__ __launch_bounds__(64) void kIoPatch(IoPatch p)
+{
+    const int i = threadIdx.x;
+    if (i >= p.n) return;
+    p.table[p.idx[i]] = p.io[i];
+}
+
 } // namespace
+
+hipError_t launchIoPatch(const IoPatch& p, hipStream_t s)
+{
+    if (p.n <= 0) return hipSuccess;
+    if (p.n > kMaxIoPatch) return hipErrorInvalidValue;
+    kIoPatch<<<1, 64, 0, s>>>(p);
+    return hipGetLastError();
+}
 
 hipError_t launchImgPatch(const ImgPatch& p, hipStream_t s)
 {

@@ -11,10 +11,13 @@
 // strict LR check :855-889 (pyramid), median :891-920 (in-bounds neighbours,
 // sorted[n/2]), confidence clear :922-931, float conversion :1530-1551, output
 // routing :1837-1858 / :2053-2069.
+#include <string.h>
+
 #include <type_traits>
 
 #include "device_common.h"
 #include "internal.h"
+#include "stage_impl.h"
 
 namespace pyr {
 namespace {
@@ -101,9 +104,8 @@ __device__ __forceinline__ int median9(int p[9])
     return p[4];
 }
 
-__global__ __launch_bounds__(256) void kPost(PostArgs args)
+__device__ __forceinline__ void kPostBody(const PostArgs& args, Tile3 tl)
 {
-    const Tile3 tl = xcdTile();
     constexpr int HW = kPostW + 2, HALO = HW * (kPostH + 2), NH = (HALO + 255) / 256;
     // This pair's buffers (pair = tl.z).
     PostArgs a = args;
@@ -231,11 +233,58 @@ __global__ __launch_bounds__(256) void kPost(PostArgs args)
         finish(std::false_type{});
 }
 
+__device__ __forceinline__ PostArgs postArgs(const StageArgs& s)
+{
+    PostArgs a;
+    a.disp = s.disp;
+    a.conf = s.conf;
+    a.keys = s.keys;
+    a.W = s.W;
+    a.H = s.H;
+    a.minD = s.minD;
+    a.maxDiff = s.maxDiff;
+    a.lrcheck = s.lrcheck;
+    a.median = s.median;
+    a.strict = s.strict;
+    a.pyramid = s.strict;
+    a.out16 = s.out16;
+    a.out32 = s.out32;
+    a.outc = s.outc;
+    a.s16 = s.s16;
+    a.s32 = s.s32;
+    a.sc = s.sc;
+    a.batch = s.batch;
+    return a;
+}
+
+__global__ __launch_bounds__(256) void kPost(StageArgs a) { kPostBody(postArgs(a), xcdTile()); }
+
+__global__ __launch_bounds__(256) void kPostQ(StageQueue q)
+{
+    runQueuedStage(q, [](const StageArgs& a, Tile3 tl) { kPostBody(postArgs(a), tl); });
+}
+
 } // namespace
+
+hipError_t launchPostStage(const StageLaunch& l, hipStream_t s)
+{
+    kPost<<<l.grid, 256, 0, s>>>(l.a);
+    return hipGetLastError();
+}
+
+hipError_t launchPostQueued(const StageQueue& q, int blocks, hipStream_t s)
+{
+    kPostQ<<<blocks, 256, 0, s>>>(q);
+    return hipGetLastError();
+}
 
 hipError_t launchPost(const LevelCall& c, Workspace& ws, hipStream_t s)
 {
-    PostArgs a;
+    StageLaunch l;
+    memset(&l, 0, sizeof(l));
+    l.kind = kStPost;
+    l.grid = dim3((c.W + kPostW - 1) / kPostW, (c.H + kPostH - 1) / kPostH, c.batch.n);
+    StageArgs& a = l.a;
     a.disp = ws.disp;
     a.conf = ws.conf;
     a.keys = ws.keys;
@@ -246,18 +295,21 @@ hipError_t launchPost(const LevelCall& c, Workspace& ws, hipStream_t s)
     a.lrcheck = (c.post & 1) != 0;
     a.median = (c.post & 2) != 0;
     a.strict = c.mode != kSingle;
-    a.pyramid = c.mode != kSingle;
     a.out16 = c.out16;
     a.out32 = c.out32;
     a.outc = c.outc;
     a.s16 = c.s16;
     a.s32 = c.s32;
     a.sc = c.sc;
-    if (!a.out16 && !a.outc) return hipSuccess;
     a.batch = c.batch;
-    dim3 grid((c.W + kPostW - 1) / kPostW, (c.H + kPostH - 1) / kPostH, c.batch.n);
-    kPost<<<grid, 256, 0, s>>>(a);
-    return hipGetLastError();
+    // A recorded launch is kept even without outputs: its queued form takes
+    // the outputs of each call from the pair's I/O entry.
+    if (c.stageSink) {
+        c.stageSink->launches.push_back(l);
+        return hipSuccess;
+    }
+    if (!a.out16 && !a.outc) return hipSuccess;
+    return launchStage(l, s);
 }
 
 } // namespace pyr

@@ -483,6 +483,7 @@ int pyrsgm_mixed_set_tuning(void* batch, const pyrsgm_mixed_tuning_c* t, char* e
         m.branches = t->branches;
         m.blocks_per_cu = t->blocks_per_cu;
         m.paired = t->paired;
+        m.queued_stages = t->queued_stages;
         static_cast<pyrsgm::MixedBatch*>(batch)->setTuning(m);
         return 0;
     } catch (const std::exception& e) {
@@ -498,6 +499,7 @@ int pyrsgm_mixed_get_tuning(void* batch, pyrsgm_mixed_tuning_c* out)
     out->branches = m.branches;
     out->blocks_per_cu = m.blocks_per_cu;
     out->paired = m.paired;
+    out->queued_stages = m.queued_stages;
     return 0;
 }
 
@@ -511,6 +513,7 @@ int pyrsgm_mixed_tune(void* batch, const pyrsgm_pair_io_c* io, hipStream_t strea
             best->branches = m.branches;
             best->blocks_per_cu = m.blocks_per_cu;
             best->paired = m.paired;
+            best->queued_stages = m.queued_stages;
         }
         return 0;
     } catch (const std::exception& e) {
@@ -519,11 +522,11 @@ int pyrsgm_mixed_tune(void* batch, const pyrsgm_pair_io_c* io, hipStream_t strea
     return -1;
 }
 
-int pyrsgm_mixed_plan(void* batch, long long out[4])
+int pyrsgm_mixed_plan(void* batch, long long out[6])
 {
     if (!batch || !out) return -1;
     const std::vector<long long> p = static_cast<pyrsgm::MixedBatch*>(batch)->plan();
-    for (int i = 0; i < 4; ++i) out[i] = p[(size_t)i];
+    for (int i = 0; i < 6; ++i) out[i] = p[(size_t)i];
     return 0;
 }
 

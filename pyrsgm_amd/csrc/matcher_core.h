@@ -267,7 +267,8 @@ struct MatcherCore {
     // Single-scale matcher (levels == 0): the given stages of the one level.
     void singleCall(BatchBuffer<const uint8_t> L, BatchBuffer<const uint8_t> R,
                     BatchBuffer<int16_t> d16, BatchBuffer<float> d32, BatchBuffer<float> oc,
-                    hipStream_t stream, int stages, int count, pyr::PathSink* sink = nullptr)
+                    hipStream_t stream, int stages, int count, pyr::PathSink* sink = nullptr,
+                    pyr::StageSink* stageSink = nullptr)
     {
         const int costMode = params.cost_mode == CostMode::Census ? PYRSGM_COST_CENSUS : PYRSGM_COST_BT;
         pyr::Batch b;
@@ -284,14 +285,14 @@ struct MatcherCore {
             params.uniqueness_ratio, params.disp12_max_diff, params.pre_filter_cap, costMode,
             postFlags(0), params.num_paths, params.adaptive_p2 ? 1 : 0, params.p2_alpha,
             params.p2_beta, params.p2_lower_mult, params.p2_upper_mult, b, stream, stages, sink,
-            &tiles);
+            &tiles, stageSink);
         if (ret != 0) throwHip("single-scale SGM", hipGetLastError());
     }
 
     // The pyramid images of levels 1..levels: both views in one launch per
     // level when their pair strides agree.
     void pyrDownChain(BatchBuffer<const uint8_t> L, BatchBuffer<const uint8_t> R,
-                      hipStream_t stream, int count)
+                      hipStream_t stream, int count, pyr::StageSink* stageSink = nullptr)
     {
         for (int level = 1; level <= levels; ++level) {
             const LevelInfo& parent = info[level - 1];
@@ -311,17 +312,17 @@ struct MatcherCore {
                                             parent.height, pyrLeft[level].data,
                                             (int)pyrLeft[level].pitch, pyrRight[level].data,
                                             (int)pyrRight[level].pitch, ds, li.width, li.height,
-                                            count, stream);
+                                            count, stream, stageSink);
             } else {
                 e = pyr::launchPyrDownBatch(sl, (int)slp, nullptr, 0, sls, parent.width,
                                             parent.height, pyrLeft[level].data,
                                             (int)pyrLeft[level].pitch, nullptr, 0, ds, li.width,
-                                            li.height, count, stream);
+                                            li.height, count, stream, stageSink);
                 if (e == hipSuccess)
                     e = pyr::launchPyrDownBatch(sr, (int)srp, nullptr, 0, srs, parent.width,
                                                 parent.height, pyrRight[level].data,
                                                 (int)pyrRight[level].pitch, nullptr, 0, ds,
-                                                li.width, li.height, count, stream);
+                                                li.width, li.height, count, stream, stageSink);
             }
             pyr::profEnd(stream);
             if (e != hipSuccess) throwHip("pyrDown", e);
@@ -331,7 +332,8 @@ struct MatcherCore {
     // One pyramid level (or the given stages of it) for the first `count` pairs.
     void levelCall(int level, BatchBuffer<const uint8_t> L, BatchBuffer<const uint8_t> R,
                    BatchBuffer<int16_t> d16, BatchBuffer<float> d32, BatchBuffer<float> oc,
-                   hipStream_t stream, int stages, int count, pyr::PathSink* sink = nullptr)
+                   hipStream_t stream, int stages, int count, pyr::PathSink* sink = nullptr,
+                   pyr::StageSink* stageSink = nullptr)
     {
         const int costMode = params.cost_mode == CostMode::Census ? PYRSGM_COST_CENSUS : PYRSGM_COST_BT;
         {
@@ -368,7 +370,7 @@ struct MatcherCore {
                 params.disp12_max_diff, params.pre_filter_cap, costMode, postFlags(level),
                 params.num_paths, refine ? 1 : 0, params.pyramid_min_confidence,
                 params.adaptive_p2 ? 1 : 0, params.p2_alpha, params.p2_beta, params.p2_lower_mult,
-                params.p2_upper_mult, b, stream, stages, sink, &tiles);
+                params.p2_upper_mult, b, stream, stages, sink, &tiles, stageSink);
             if (ret != 0) {
                 std::ostringstream oss;
                 oss << "pyramid level " << level << " (" << li.width << "x" << li.height

@@ -2,6 +2,7 @@
 // one fused pass (queued path and winner-take-all launches over all pairs,
 // per-pair stages on parallel streams, whole pass capturable as one HIP
 // graph), and its tuner.
+#include <stdint.h>
 #include <string.h>
 
 #include <algorithm>
@@ -35,6 +36,24 @@ void checkTuning(const MixedTuning& t)
         throw std::invalid_argument("pyrsgm: MixedTuning.blocks_per_cu must be in [0, 64]");
     if (t.paired < -1 || t.paired > 1)
         throw std::invalid_argument("pyrsgm: MixedTuning.paired must be -1, 0 or 1");
+    if (t.queued_stages != 0 && t.queued_stages != 1)
+        throw std::invalid_argument("pyrsgm: MixedTuning.queued_stages must be 0 or 1");
+}
+
+// Phases of a pyramid step's stage launches, in issue order: the cost pass
+// that follows another cost pass of the same pair (the BT vertical window sums
+// after the row sums) runs in a phase of its own.
+enum StagePhase : int { kPhPyrDown = 0, kPhPrep, kPhCost, kPhCost2, kPhPost };
+int phaseOf(int kind)
+{
+    switch (kind) {
+    case pyr::kStPyrDown: return kPhPyrDown;
+    case pyr::kStPrepCensus:
+    case pyr::kStPrepBT: return kPhPrep;
+    case pyr::kStBoxV: return kPhCost2;
+    case pyr::kStPost: return kPhPost;
+    default: return kPhCost;
+    }
 }
 
 } // namespace
@@ -53,12 +72,33 @@ struct MixedBatch::Impl {
         int pair, level;
         pyr::PathTask t;
     };
+    // A recorded stage launch of one pair: pyramid level (or pyrDown depth,
+    // with level -1), its launch and its I/O flags.
+    struct RecordedStage {
+        int pair, level, depth;
+        pyr::StageLaunch l;
+    };
+    // One queued stage launch: the tasks of one step and phase that share a
+    // stage kernel (pyrDown: one per depth, step = depth - 1).
+    struct StageGroup {
+        int step = 0, phase = 0, kind = 0, variant = 0;
+        int taskOff = 0, nTasks = 0, blocks = 0;
+        size_t lds = 0, startOff = 0;
+    };
 
     std::vector<PairSpec> specs;
     std::vector<std::unique_ptr<MatcherCore>> cores;
     int maxSteps = 0;
     MixedTuning tun;
     std::vector<Recorded> recorded;       // every pair level's path task, as recorded
+    std::vector<RecordedStage> recordedStages;  // every other launch of every pair, as recorded
+    std::vector<StageGroup> stageGroups;        // in issue order
+    std::vector<pyr::StageTask> hostStageTasks;
+    std::vector<int> hostStageStart;
+    pyr::StageTask* dStageTasks = nullptr;
+    int* dStageStart = nullptr;
+    pyr::StageIo* dIo = nullptr;
+    long long stageLaunches = 0;
     std::vector<Group> groups;
     std::vector<pyr::PathArgs> hostTasks;  // device table image
     std::vector<int> hostStart;            // per group nTasks + 1 entries, concatenated
@@ -159,8 +199,99 @@ struct MixedBatch::Impl {
                     c.levelCall(level, img, img, d16, {}, {}, nullptr, pyr::kStagePaths, 1, &sink);
                 if (sink.tasks.size() != 1) throw std::logic_error("pyrsgm: path task not recorded");
                 recorded.push_back({i, level, sink.tasks[0]});
+                // The pair's other launches of this level; level 0 takes the
+                // caller's images (and writes the caller's outputs) per call.
+                pyr::StageSink ss;
+                const int st = pyr::kStagePrep | pyr::kStageCost | pyr::kStagePost;
+                if (c.levels == 0)
+                    c.singleCall(img, img, d16, {}, {}, nullptr, st, 1, nullptr, &ss);
+                else
+                    c.levelCall(level, img, img, d16, {}, {}, nullptr, st, 1, nullptr, &ss);
+                for (pyr::StageLaunch& l : ss.launches) {
+                    if (level == 0) l.io = l.kind == pyr::kStPost ? pyr::kIoOut : pyr::kIoIn;
+                    recordedStages.push_back({i, level, 0, l});
+                }
+            }
+            if (c.levels > 0) {
+                pyr::StageSink ss;
+                c.pyrDownChain(img, img, nullptr, 1, &ss);
+                if ((int)ss.launches.size() != c.levels) throw std::logic_error("pyrsgm: pyrDown launches not recorded");
+                for (int k = 0; k < c.levels; ++k) {
+                    pyr::StageLaunch& l = ss.launches[(size_t)k];
+                    if (k == 0) l.io = pyr::kIoIn;  // the first reduction reads the caller's images
+                    recordedStages.push_back({i, -1, k + 1, l});
+                }
             }
         }
+    }
+
+    // The queued stage launches: every recorded stage launch of one step (or
+    // pyrDown depth) and phase, grouped by kernel, largest tasks first.
+    void planStages()
+    {
+        stageGroups.clear();
+        hostStageTasks.clear();
+        hostStageStart.clear();
+        stageLaunches = 0;
+        if (!tun.queued_stages) return;
+        auto addGroups = [&](int step, int phase, const std::vector<int>& members) {
+            std::map<std::pair<int, int>, std::vector<int>> byKernel;
+            for (int r : members) byKernel[{recordedStages[r].l.kind, recordedStages[r].l.variant}].push_back(r);
+            for (auto& kv : byKernel) {
+                std::vector<int>& rs = kv.second;
+                auto blocksOf = [&](int r) {
+                    const dim3& g = recordedStages[r].l.grid;
+                    return (long long)g.x * g.y * g.z;
+                };
+                std::stable_sort(rs.begin(), rs.end(), [&](int x, int y) { return blocksOf(x) > blocksOf(y); });
+                StageGroup g;
+                g.step = step;
+                g.phase = phase;
+                g.kind = kv.first.first;
+                g.variant = kv.first.second;
+                g.taskOff = (int)hostStageTasks.size();
+                g.startOff = hostStageStart.size();
+                long long total = 0;
+                for (int r : rs) {
+                    const RecordedStage& rec = recordedStages[r];
+                    pyr::StageTask t;
+                    memset(&t, 0, sizeof(t));
+                    t.a = rec.l.a;
+                    t.gx = (int)rec.l.grid.x;
+                    t.gy = (int)rec.l.grid.y;
+                    t.io = rec.l.io;
+                    t.pair = rec.pair;
+                    hostStageStart.push_back((int)total);
+                    total += blocksOf(r);
+                    g.lds = std::max(g.lds, rec.l.lds);
+                    hostStageTasks.push_back(t);
+                }
+                if (total > INT32_MAX) throw std::length_error("pyrsgm: queued stage launch too large");
+                hostStageStart.push_back((int)total);
+                g.nTasks = (int)rs.size();
+                g.blocks = (int)total;
+                stageGroups.push_back(g);
+                ++stageLaunches;
+            }
+        };
+        int maxDepth = 0;
+        for (auto& c : cores) maxDepth = std::max(maxDepth, c->levels);
+        for (int depth = 1; depth <= maxDepth; ++depth) {
+            std::vector<int> m;
+            for (int r = 0; r < (int)recordedStages.size(); ++r)
+                if (recordedStages[r].depth == depth) m.push_back(r);
+            addGroups(depth - 1, kPhPyrDown, m);
+        }
+        for (int step = 0; step < maxSteps; ++step)
+            for (int phase : {kPhPrep, kPhCost, kPhCost2, kPhPost}) {
+                std::vector<int> m;
+                for (int r = 0; r < (int)recordedStages.size(); ++r) {
+                    const RecordedStage& rec = recordedStages[r];
+                    if (rec.depth == 0 && levelAt(rec.pair, step) == rec.level && phaseOf(rec.l.kind) == phase)
+                        m.push_back(r);
+                }
+                addGroups(step, phase, m);
+            }
     }
 
     // Groups, task tables and queue counters for the current tuning.
@@ -243,16 +374,25 @@ struct MixedBatch::Impl {
                 g.grid = (int)std::min<long long>((long long)total, (long long)numCUs * bpc);
                 groups.push_back(std::move(g));
             }
-            for (int i = 0; i < (int)cores.size(); ++i)
-                if (levelAt(i, step) >= 0) perPairLaunches += 3;  // prep, cost, post
+            if (!tun.queued_stages)
+                for (int i = 0; i < (int)cores.size(); ++i)
+                    if (levelAt(i, step) >= 0) perPairLaunches += 3;  // prep, cost, post
         }
-        for (auto& c : cores) perPairLaunches += c->levels;  // pyrDown
-        // Device tables: tasks, starts, two counters per group.
-        const size_t tasksBytes = (hostTasks.size() * sizeof(pyr::PathArgs) + 255) / 256 * 256;
-        const size_t startBytes = (hostStart.size() * sizeof(int) + 255) / 256 * 256;
-        const size_t wtaBytes = (hostWtaStart.size() * sizeof(int) + 255) / 256 * 256;
-        const size_t counterBytes = groups.size() * 2 * sizeof(uint32_t);
-        const size_t need = tasksBytes + startBytes + wtaBytes + counterBytes;
+        if (!tun.queued_stages)
+            for (auto& c : cores) perPairLaunches += c->levels;  // pyrDown
+        planStages();
+        // Device tables: path tasks, starts, two counters per group; stage
+        // tasks, their starts and the per-pair I/O table.
+        auto round = [](size_t b) { return (b + 255) / 256 * 256; };
+        const size_t tasksBytes = round(hostTasks.size() * sizeof(pyr::PathArgs));
+        const size_t startBytes = round(hostStart.size() * sizeof(int));
+        const size_t wtaBytes = round(hostWtaStart.size() * sizeof(int));
+        const size_t counterBytes = round(groups.size() * 2 * sizeof(uint32_t));
+        const size_t stageBytes = round(hostStageTasks.size() * sizeof(pyr::StageTask));
+        const size_t stageStartBytes = round(hostStageStart.size() * sizeof(int));
+        const size_t ioBytes = round(cores.size() * sizeof(pyr::StageIo));
+        const size_t need = tasksBytes + startBytes + wtaBytes + counterBytes + stageBytes +
+                            stageStartBytes + ioBytes;
         if (need > devBytes) {
             if (dev) check(hipFree(dev), "hipFree");
             dev = nullptr;
@@ -265,6 +405,17 @@ struct MixedBatch::Impl {
         dStart = reinterpret_cast<int*>(base + tasksBytes);
         dWtaStart = reinterpret_cast<int*>(base + tasksBytes + startBytes);
         dCounters = reinterpret_cast<uint32_t*>(base + tasksBytes + startBytes + wtaBytes);
+        char* const stageBase = base + tasksBytes + startBytes + wtaBytes + counterBytes;
+        dStageTasks = reinterpret_cast<pyr::StageTask*>(stageBase);
+        dStageStart = reinterpret_cast<int*>(stageBase + stageBytes);
+        dIo = reinterpret_cast<pyr::StageIo*>(stageBase + stageBytes + stageStartBytes);
+        if (!hostStageTasks.empty()) {
+            check(hipMemcpy(dStageTasks, hostStageTasks.data(), hostStageTasks.size() * sizeof(pyr::StageTask),
+                            hipMemcpyHostToDevice), "hipMemcpy");
+            check(hipMemcpy(dStageStart, hostStageStart.data(), hostStageStart.size() * sizeof(int),
+                            hipMemcpyHostToDevice), "hipMemcpy");
+        }
+        check(hipMemset(dIo, 0, ioBytes), "hipMemset");
         check(hipMemcpy(dTasks, hostTasks.data(), hostTasks.size() * sizeof(pyr::PathArgs),
                         hipMemcpyHostToDevice), "hipMemcpy");
         check(hipMemcpy(dStart, hostStart.data(), hostStart.size() * sizeof(int), hipMemcpyHostToDevice),
@@ -341,9 +492,120 @@ struct MixedBatch::Impl {
         }
     }
 
+    // The queued path and winner-take-all launches of one step (in compute()
+    // and computeQueued()).
+    void pathGroups(const pyrsgm_pair_io_c* io, hipStream_t main, int step, size_t& gi)
+    {
+        for (; gi < groups.size() && groups[gi].step == step; ++gi) {
+            const Group& g = groups[gi];
+            for (size_t p0 = 0; p0 < g.patchTask.size(); p0 += pyr::kMaxImgPatches) {
+                pyr::ImgPatch patch;
+                memset(&patch, 0, sizeof(patch));
+                patch.tasks = dTasks;
+                patch.n = (int)std::min<size_t>(pyr::kMaxImgPatches, g.patchTask.size() - p0);
+                for (int k = 0; k < patch.n; ++k) {
+                    const int pr = g.patchPair[p0 + k];
+                    patch.idx[k] = g.patchTask[p0 + k];
+                    patch.img[k] = io[pr].left;
+                    patch.istep[k] = (int)io[pr].left_pitch;
+                }
+                check(pyr::launchImgPatch(patch, main), "image patch");
+            }
+            pyr::QueueArgs q;
+            q.tasks = dTasks + g.taskOff;
+            q.start = dStart + startOff[gi];
+            q.counter = dCounters + 2 * gi;
+            q.nTasks = g.nTasks;
+            q.total = g.total;
+            pyr::profBegin(PYRSGM_STAGE_PATHS, 0, 0, g.Dp, main);
+            const hipError_t e = pyr::launchPathsQueued(q, g.mode, g.Dp, g.adaptive, g.exact,
+                                                        g.narrow, g.paired, g.grid, main);
+            pyr::profEnd(main);
+            if (e != hipSuccess) {
+                std::ostringstream oss;
+                oss << "queued paths (step " << step << ", labels " << g.Dp << ", " << g.nTasks << " tasks)";
+                throwHip(oss.str(), e);
+            }
+            pyr::WtaQueueArgs wq;
+            wq.tasks = dTasks + g.taskOff;
+            wq.start = dWtaStart + wtaStartOff[gi];
+            wq.nTasks = g.nTasks;
+            wq.total = g.wtaTotal;
+            pyr::profBegin(PYRSGM_STAGE_WTA, 0, 0, g.Dp, main);
+            const hipError_t ew = pyr::launchWTAQueued(wq, g.mode, g.Dp, g.narrow, main);
+            pyr::profEnd(main);
+            if (ew != hipSuccess) throwHip("queued winner-take-all", ew);
+        }
+    }
+
+    // The whole pass as queued launches on one stream: the caller's buffers
+    // into the I/O table, the pyrDown depths, then per step the descriptor,
+    // cost, path, winner-take-all and post launches, each over every pair
+    // that has a level at that step.
+    void computeQueued(const pyrsgm_pair_io_c* io, hipStream_t main)
+    {
+        const int n = (int)cores.size();
+        for (int p0 = 0; p0 < n; p0 += pyr::kMaxIoPatch) {
+            pyr::IoPatch patch;
+            memset(&patch, 0, sizeof(patch));
+            patch.table = dIo;
+            patch.n = std::min(pyr::kMaxIoPatch, n - p0);
+            for (int k = 0; k < patch.n; ++k) {
+                const int i = p0 + k;
+                const MatcherCore& c = *cores[i];
+                const BatchBuffer<float> oc = c.confRoute({io[i].disp32, io[i].disp32_pitch, 0},
+                                                          {io[i].conf, io[i].conf_pitch, 0});
+                pyr::StageIo& x = patch.io[k];
+                x.left = io[i].left;
+                x.right = io[i].right;
+                x.lstep = io[i].left_pitch;
+                x.rstep = io[i].right_pitch;
+                x.out16 = io[i].disp16;
+                x.s16 = io[i].disp16_pitch;
+                x.out32 = io[i].disp32;
+                x.s32 = io[i].disp32_pitch;
+                x.outc = oc.data;
+                x.sc = oc.pitch;
+                patch.idx[k] = i;
+            }
+            check(pyr::launchIoPatch(patch, main), "I/O table");
+        }
+        size_t sg = 0, gi = 0;
+        auto stages = [&](int step, int lastPhase) {
+            for (; sg < stageGroups.size() && stageGroups[sg].step == step && stageGroups[sg].phase <= lastPhase; ++sg) {
+                const StageGroup& g = stageGroups[sg];
+                pyr::StageQueue q;
+                q.tasks = dStageTasks + g.taskOff;
+                q.start = dStageStart + g.startOff;
+                q.io = dIo;
+                q.nTasks = g.nTasks;
+                const hipError_t e = pyr::launchStageQueued(g.kind, g.variant, q, g.blocks, g.lds, main);
+                if (e != hipSuccess) {
+                    std::ostringstream oss;
+                    oss << "queued stage (kind " << g.kind << ", step " << step << ", " << g.nTasks << " tasks)";
+                    throwHip(oss.str(), e);
+                }
+            }
+        };
+        // pyrDown groups carry step = depth - 1 and come first.
+        while (sg < stageGroups.size() && stageGroups[sg].phase == kPhPyrDown) {
+            const int d = stageGroups[sg].step;
+            stages(d, kPhPyrDown);
+        }
+        for (int step = 0; step < maxSteps; ++step) {
+            stages(step, kPhCost2);
+            pathGroups(io, main, step, gi);
+            stages(step, kPhPost);
+        }
+    }
+
     void compute(const pyrsgm_pair_io_c* io, hipStream_t main)
     {
         checkIO(io);
+        if (tun.queued_stages) {
+            computeQueued(io, main);
+            return;
+        }
         const int n = (int)cores.size();
         auto br = [&](int i) { return tun.branches > 1 ? branch[branchOf[i]] : main; };
         fork(main);
@@ -357,46 +619,7 @@ struct MixedBatch::Impl {
                 if (levelAt(i, step) >= 0)
                     pairStages(i, levelAt(i, step), io[i], br(i), pyr::kStagePrep | pyr::kStageCost);
             join(main);
-            for (; gi < groups.size() && groups[gi].step == step; ++gi) {
-                const Group& g = groups[gi];
-                for (size_t p0 = 0; p0 < g.patchTask.size(); p0 += pyr::kMaxImgPatches) {
-                    pyr::ImgPatch patch;
-                    memset(&patch, 0, sizeof(patch));
-                    patch.tasks = dTasks;
-                    patch.n = (int)std::min<size_t>(pyr::kMaxImgPatches, g.patchTask.size() - p0);
-                    for (int k = 0; k < patch.n; ++k) {
-                        const int pr = g.patchPair[p0 + k];
-                        patch.idx[k] = g.patchTask[p0 + k];
-                        patch.img[k] = io[pr].left;
-                        patch.istep[k] = (int)io[pr].left_pitch;
-                    }
-                    check(pyr::launchImgPatch(patch, main), "image patch");
-                }
-                pyr::QueueArgs q;
-                q.tasks = dTasks + g.taskOff;
-                q.start = dStart + startOff[gi];
-                q.counter = dCounters + 2 * gi;
-                q.nTasks = g.nTasks;
-                q.total = g.total;
-                pyr::profBegin(PYRSGM_STAGE_PATHS, 0, 0, g.Dp, main);
-                const hipError_t e = pyr::launchPathsQueued(q, g.mode, g.Dp, g.adaptive, g.exact,
-                                                            g.narrow, g.paired, g.grid, main);
-                pyr::profEnd(main);
-                if (e != hipSuccess) {
-                    std::ostringstream oss;
-                    oss << "queued paths (step " << step << ", labels " << g.Dp << ", " << g.nTasks << " tasks)";
-                    throwHip(oss.str(), e);
-                }
-                pyr::WtaQueueArgs wq;
-                wq.tasks = dTasks + g.taskOff;
-                wq.start = dWtaStart + wtaStartOff[gi];
-                wq.nTasks = g.nTasks;
-                wq.total = g.wtaTotal;
-                pyr::profBegin(PYRSGM_STAGE_WTA, 0, 0, g.Dp, main);
-                const hipError_t ew = pyr::launchWTAQueued(wq, g.mode, g.Dp, g.narrow, main);
-                pyr::profEnd(main);
-                if (ew != hipSuccess) throwHip("queued winner-take-all", ew);
-            }
+            pathGroups(io, main, step, gi);
             fork(main);
             for (int i = 0; i < n; ++i)
                 if (levelAt(i, step) >= 0) pairStages(i, levelAt(i, step), io[i], br(i), pyr::kStagePost);
@@ -474,12 +697,20 @@ struct MixedBatch::Impl {
                 best = t;
             }
         };
-        for (int b : {1, 2, 4}) {
+        for (int qs : {1, 0}) {
             MixedTuning t = best;
-            if (t.branches == b) continue;
-            t.branches = b;
+            if (t.queued_stages == qs) continue;
+            t.queued_stages = qs;
             consider(t);
         }
+        // Streams matter only to the per-pair stage launches.
+        if (!best.queued_stages)
+            for (int b : {1, 2, 4}) {
+                MixedTuning t = best;
+                if (t.branches == b) continue;
+                t.branches = b;
+                consider(t);
+            }
         for (int bpc : {0, 1, 2, 4, 8}) {
             MixedTuning t = best;
             if (t.blocks_per_cu == bpc) continue;
@@ -510,7 +741,8 @@ MixedBatch::MixedBatch(const std::vector<PairSpec>& pairs, const MixedTuning& tu
     std::lock_guard<std::mutex> lock(g_tuneMutex);
     auto it = g_tuned.find(impl_->signature());
     if (it != g_tuned.end() && tuning.branches == MixedTuning{}.branches &&
-        tuning.blocks_per_cu == MixedTuning{}.blocks_per_cu && tuning.paired == MixedTuning{}.paired) {
+        tuning.blocks_per_cu == MixedTuning{}.blocks_per_cu && tuning.paired == MixedTuning{}.paired &&
+        tuning.queued_stages == MixedTuning{}.queued_stages) {
         impl_->tun = it->second;
         impl_->plan();
     }
@@ -546,7 +778,8 @@ std::vector<long long> MixedBatch::plan() const
         tasks += g.nTasks;
         items += g.total;
     }
-    return {(long long)impl_->groups.size(), tasks, items, impl_->perPairLaunches};
+    return {(long long)impl_->groups.size(), tasks, items, impl_->perPairLaunches,
+            impl_->stageLaunches, (long long)impl_->hostStageTasks.size()};
 }
 
 } // namespace pyrsgm

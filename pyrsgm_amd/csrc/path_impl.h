@@ -323,13 +323,15 @@ struct Lds {
                   "a compute lane's chunk of costs is CQ pieces");
     // Bytes per compute lane in a cost slot: 16 bytes of padding keep the
     // lanes' 16-byte reads on distinct banks.  Byte slots are unpadded (16-byte
-    // reads at a 16-byte stride are conflict-free) and each line's 16 lanes are
-    // followed by LPAD bytes, which spreads the memory wave's byte writes of
-    // different lines over different banks.
+    // reads at a 16-byte stride are conflict-free) and every group of 16 lanes
+    // is followed by LPAD bytes: the memory wave writes byte t of a pixel's
+    // 16-label piece to lane d0 + t of one group, and without the skew the
+    // groups of one line (d0 = 0, 16, 32, 48) or of four lines would hit the
+    // same banks at every t.
     static constexpr int LANE = NARROW ? 16 * CQ : CQ * 16 + 16;
     static constexpr int LPAD = NARROW ? 16 : 0;
-    static constexpr int CSLOT = 64 * LANE + LPW * LPAD;
-    __device__ static constexpr int laneOff(int l) { return l * LANE + (l / G) * LPAD; }
+    static constexpr int CSLOT = 64 * LANE + 4 * LPAD;
+    __device__ static constexpr int laneOff(int l) { return l * LANE + (l / 16) * LPAD; }
     // Per-step words, [line][step slot]: the memory wave writes 64 per lane set
     // (two sets for paired lines).
     static constexpr int WSLOT = 64 * 4 * (PK ? 2 : 1);

@@ -98,6 +98,13 @@ def test_mixed_eager_equals_alone(scene):
     pair_levels = sum(s[3] + 1 for s in SPECS)
     assert plan["path_tasks"] == pair_levels
     assert plan["queued_path_launches"] < pair_levels  # tasks of one kernel and step share a launch
+    # Every other stage is queued too: no per-pair launch is left, and each pair
+    # level contributes its descriptor pass, cost pass(es) and post as tasks,
+    # plus one pyrDown task per pyramid level.
+    assert plan["per_pair_stage_calls"] == 0
+    pyr_levels = sum(s[3] for s in SPECS)
+    assert plan["stage_tasks"] >= 3 * pair_levels + pyr_levels
+    assert plan["queued_stage_launches"] < plan["stage_tasks"]
     io = make_io(pairs)
     print("case: mixed eager,", len(SPECS), "pairs, plan", plan, flush=True)
     mb.compute(tensors(io))
@@ -108,27 +115,34 @@ def test_mixed_eager_equals_alone(scene):
     mb.close()
 
 
-@pytest.mark.parametrize("branches,bpc,paired", [(1, 0, -1), (2, 1, 0), (4, 2, 1), (3, 8, -1)])
-def test_mixed_tunings_equal_alone(scene, branches, bpc, paired):
+@pytest.mark.parametrize("branches,bpc,paired,queued", [(1, 0, -1, 1), (2, 1, 0, 0), (4, 2, 1, 1),
+                                                       (3, 8, -1, 0), (4, 0, -1, 0)])
+def test_mixed_tunings_equal_alone(scene, branches, bpc, paired, queued):
     pairs, specs, refs = scene
     mb = P.MixedBatch(specs)
-    mb.set_tuning(branches, bpc, paired)
-    assert mb.tuning() == {"branches": branches, "blocks_per_cu": bpc, "paired": paired}
+    mb.set_tuning(branches, bpc, paired, queued)
+    assert mb.tuning() == {"branches": branches, "blocks_per_cu": bpc, "paired": paired,
+                           "queued_stages": queued}
+    plan = mb.plan()
+    assert (plan["per_pair_stage_calls"] == 0) == bool(queued)
+    assert (plan["queued_stage_launches"] > 0) == bool(queued)
     io = make_io(pairs)
-    print("case: mixed tuning", branches, bpc, paired, flush=True)
+    print("case: mixed tuning", branches, bpc, paired, queued, flush=True)
     for rep in range(2):
         reset(io)
         mb.compute(tensors(io))
         U.sync()
-        check(io, refs, f"tuning {branches}/{bpc}/{paired} run {rep}")
+        check(io, refs, f"tuning {branches}/{bpc}/{paired}/{queued} run {rep}")
     mb.close()
 
 
-def test_mixed_graph_replays_equal_alone(scene):
+@pytest.mark.parametrize("queued", [1, 0])
+def test_mixed_graph_replays_equal_alone(scene, queued):
     pairs, specs, refs = scene
     mb = P.MixedBatch(specs)
+    mb.set_tuning(queued_stages=queued)
     io = make_io(pairs)
-    print("case: mixed graph", flush=True)
+    print("case: mixed graph, queued stages", queued, flush=True)
     mb.capture(tensors(io))
     junk = torch.zeros(1 << 20, device="cuda")
     for rep in range(3):
@@ -152,6 +166,7 @@ def test_mixed_tune_keeps_results(scene):
     assert ms > 0
     assert mb.tuning() == best
     assert 1 <= best["branches"] <= 4 and best["paired"] in (-1, 0, 1)
+    assert best["queued_stages"] in (0, 1)
     reset(io)
     mb.compute(tensors(io))
     U.sync()
@@ -174,6 +189,8 @@ def test_mixed_rejections(scene):
         mb.compute([t[0], (t[1][0], t[1][1], None, None, None)])  # a pair with no output
     with pytest.raises(ValueError):
         mb.set_tuning(branches=5)
+    with pytest.raises(ValueError):
+        mb.set_tuning(queued_stages=2)
     with pytest.raises(RuntimeError):
         mb.launch()  # nothing captured
     mb.close()

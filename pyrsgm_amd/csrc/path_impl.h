@@ -55,6 +55,15 @@ constexpr uint32_t kOffLimit = 0x80000000u;  // offset given to lanes of absent 
 template <int K>
 constexpr int prefetchSteps() { return K <= 2 ? PYRSGM_PREFETCH_STEPS : 32 / K; }
 
+// Register sets of the memory wave (compile-time tuning knob): a chunk's
+// global loads are issued this many chunks before the chunk is scattered into
+// LDS, which is two chunks before the recurrence reads it.
+#ifndef PYRSGM_MEM_SETS
+#define PYRSGM_MEM_SETS 2
+#endif
+constexpr int kMemSets = PYRSGM_MEM_SETS;
+static_assert(kMemSets >= 2, "the memory wave keeps at least two chunks of loads in flight");
+
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* base, uint32_t bytes)
 {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes,
@@ -342,11 +351,12 @@ struct Lds {
     static constexpr int NQ = PIECES / 64;
     static constexpr int OFF_W = 3 * CSLOT;
     static constexpr int OFF_T = OFF_W + 3 * WSLOT;
-    // Modular refinement step (MODR): per ring slot, three lane-mask slots laid
-    // out like a cost slot (a byte per compute lane and step), and per chunk
-    // (4 in flight) the rotation of each of the memory wave's pieces.
+    // Modular refinement step (MODR): per ring slot, two mask words per (line,
+    // step) laid out like the per-step words (M0 | MM << 16, then MP: 16-bit
+    // lane masks), and per chunk (4 in flight) the rotation of each of the
+    // memory wave's pieces.
     static constexpr int OFF_M = OFF_T + 2 * TSLOT;
-    static constexpr int MSLOT = MODR ? 3 * CSLOT : 0;
+    static constexpr int MSLOT = MODR ? 2 * WSLOT : 0;
     static constexpr int OFF_R = OFF_M + 3 * MSLOT;
     static constexpr int BYTES = OFF_R + (MODR ? 4 * 64 : 0);
 };
@@ -392,35 +402,6 @@ __device__ __forceinline__ u4v rotDown16(const u4v& x, int R)
 #pragma unroll
     for (int i = 0; i < 4; ++i) o[i] = __builtin_amdgcn_alignbyte(z[(i + 1) & 3], z[i], (uint32_t)(R & 3));
     return o;
-}
-
-// 16 mask bits -> 16 bytes of 0x00 / 0xFF (bit i -> byte i).
-__device__ __forceinline__ u4v bitsToBytes16(uint32_t m)
-{
-    u4v o;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) o[i] = (((m >> (4 * i)) & 15u) * 0x00204081u & 0x01010101u) * 0xFFu;
-    return o;
-}
-
-// x & (sign-extended byte SEL of m): m's bytes are 0x00 or 0xFF.
-template <int SEL>
-__device__ __forceinline__ int andByteMask(int x, uint32_t m)
-{
-    int r;
-    if constexpr (SEL == 0)
-        asm("v_and_b32_sdwa %0, %1, sext(%2) dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_0"
-            : "=v"(r) : "v"(x), "v"(m));
-    else if constexpr (SEL == 1)
-        asm("v_and_b32_sdwa %0, %1, sext(%2) dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_1"
-            : "=v"(r) : "v"(x), "v"(m));
-    else if constexpr (SEL == 2)
-        asm("v_and_b32_sdwa %0, %1, sext(%2) dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_2"
-            : "=v"(r) : "v"(x), "v"(m));
-    else
-        asm("v_and_b32_sdwa %0, %1, sext(%2) dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_3"
-            : "=v"(r) : "v"(x), "v"(m));
-    return r;
 }
 
 // MODE: kSingle, kCoarse or kRefine.
@@ -563,12 +544,27 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
         const int nRows = VERT ? nSteps : nValid;
         const int nCols = VERT ? nValid : nSteps;
         char* const slot = s_lds + (k % 3) * TL::CSLOT;
-        int info = 0;  // MODR: lane (line, r): window shift + 32 (bits 0-7), base & 15 (bits 8-11)
+        int info = 0;  // MODR: lane (line, r): base & 15 of step r's window
         if constexpr (MODR) {
             const int base = (int)(int16_t)(m.w[0] & 0xFFFF);
             const int prevBase = dppOpaque<kDppRowShr1>(mcarry, base);
             mcarry = rowBcast<U - 1>(base);
-            info = (imin(imax(base - prevBase, -17), 17) + 32) | ((base & 15) << 8);
+            const int sh = imin(imax(base - prevBase, -17), 17);
+            const int rot = base & 15;
+            info = rot;
+            // Lane masks of step r of line j (this lane's word): the previous
+            // pixel's path cost for real base + o + e (label o, e = 0, -1, +1)
+            // exists iff o + sh + e lies in [0, 15]; label o is held by
+            // compute lane (o + rot) & 15.
+            auto range = [](int lo, int hi) {
+                const int a = imin(imax(lo, 0), 16), b = imin(imax(hi + 1, 0), 16);
+                return b > a ? ((1u << b) - 1u) & ~((1u << a) - 1u) : 0u;
+            };
+            auto toLanes = [&](uint32_t v) { return ((v | (v << 16)) >> (16 - rot)) & 0xFFFFu; };
+            char* const pm = s_lds + TL::OFF_M + (k % 3) * TL::MSLOT + lane * 4;
+            *reinterpret_cast<uint32_t*>(pm) =
+                toLanes(range(-sh, 15 - sh)) | (toLanes(range(1 - sh, 16 - sh)) << 16);
+            *reinterpret_cast<uint32_t*>(pm + TL::WSLOT) = toLanes(range(-1 - sh, 14 - sh));
         }
 #pragma unroll
         for (int i = 0; i < TL::NQ; ++i) {
@@ -578,13 +574,13 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
             const int col = ob / VB;
             int pinfo = 0;
             if constexpr (MODR) {
-                // (shift, rotation) of this piece's (line, step), from the lane
-                // that loaded that step's word (every lane takes part).
+                // Rotation of this piece's (line, step), from the lane that
+                // loaded that step's word (every lane takes part).
                 const int jj = VERT ? col : row;
                 const int u = VERT ? row : (RL ? nSteps - 1 - col : col);
                 const int src = imin(imax(jj, 0), LPW - 1) * 16 + imin(imax(u, 0), 15);
                 pinfo = __builtin_amdgcn_ds_bpermute(src << 2, info);
-                *reinterpret_cast<uint8_t*>(s_lds + TL::OFF_R + (k & 3) * 64 + pc) = (uint8_t)(pinfo >> 8);
+                *reinterpret_cast<uint8_t*>(s_lds + TL::OFF_R + (k & 3) * 64 + pc) = (uint8_t)pinfo;
             }
             if (row < nRows && col < nCols) {
                 const int jj = VERT ? col : row;
@@ -596,30 +592,7 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
                 if constexpr (MODR) {
                     // Lane g takes label (g - rot) & 15, the label whose real
                     // disparity (base + label) is congruent to g modulo 16.
-                    const int sh = (pinfo & 0xFF) - 32, rot = (pinfo >> 8) & 15;
-                    q = rotDown16(q, (16 - rot) & 15);
-                    // Masks in label order o: the previous pixel's path cost for
-                    // real base + o + e (e = 0, -1, +1) exists iff o + sh + e lies in
-                    // [0, 15]; then rotated to lane order and widened to bytes.
-                    auto range = [](int lo, int hi) {
-                        const int a = imin(imax(lo, 0), 16), b = imin(imax(hi + 1, 0), 16);
-                        return b > a ? ((1u << b) - 1u) & ~((1u << a) - 1u) : 0u;
-                    };
-                    auto toLanes = [&](uint32_t v) {
-                        return bitsToBytes16(((v | (v << 16)) >> (16 - rot)) & 0xFFFFu);
-                    };
-                    const u4v m0 = toLanes(range(-sh, 15 - sh));
-                    const u4v mm = toLanes(range(1 - sh, 16 - sh));
-                    const u4v mp = toLanes(range(-1 - sh, 14 - sh));
-                    char* const pm = s_lds + TL::OFF_M + (k % 3) * TL::MSLOT +
-                                     TL::laneOff((jj % LPW) * G) + u;
-#pragma unroll
-                    for (int t = 0; t < 16; ++t) {
-                        const int sft = 8 * (t & 3);
-                        *reinterpret_cast<uint8_t*>(pm + t * TL::LANE) = (uint8_t)(m0[t >> 2] >> sft);
-                        *reinterpret_cast<uint8_t*>(pm + TL::CSLOT + t * TL::LANE) = (uint8_t)(mm[t >> 2] >> sft);
-                        *reinterpret_cast<uint8_t*>(pm + 2 * TL::CSLOT + t * TL::LANE) = (uint8_t)(mp[t >> 2] >> sft);
-                    }
+                    q = rotDown16(q, (16 - pinfo) & 15);
                 }
                 if constexpr (NARROW) {
 #pragma unroll
@@ -768,41 +741,58 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
     };
 
     if (memWave) {
-        // Chunk c's loads are issued two iterations before it is scattered
-        // into LDS (register sets R0 / R1 by chunk parity); iteration k
-        // (between barriers k and k+1) fills slot k+2 and stores tile k-1.
-        // FUSE: iteration k finishes chunk k-1 from set (k-1) % 2 and then
-        // loads chunk k+1's vectors into that set.
-        MBuf R0, R1;
+        // Chunk c is loaded into register set c % NS and scattered into LDS
+        // slot c % 3 in iteration c - 2 (between barriers c - 2 and c - 1), so
+        // its loads are issued NS iterations before they are needed.
+        // Iteration k fills slot k+2, reloads that register set with chunk
+        // k+2+NS and stores tile k-1.  FUSE: iteration k finishes chunk k-1
+        // from pixel-vector set (k-1) % 2 and then loads chunk k+1's vectors
+        // into that set.
+        constexpr int NS = kMemSets;
+        MBuf R[NS];
+#pragma unroll
+        for (int c = 0; c < NS; ++c)
+            if (c < nChunks) mload(R[c], c);
+        mstore(R[0], 0);
+        if (NS < nChunks) mload(R[0], NS);
+        if (nChunks > 1) mstore(R[1], 1);
+        if (NS + 1 < nChunks) mload(R[1], NS + 1);
         ABuf A0, A1;
-        mload(R0, 0);
-        if (nChunks > 1) mload(R1, 1);
-        mstore(R0, 0);
-        if (nChunks > 1) mstore(R1, 1);
-        if (nChunks > 2) mload(R0, 2);
-        if (nChunks > 3) mload(R1, 3);
         aload(A0, 0);
         wgBarrier();
+        // Iteration m with m % NS == I (a compile-time constant, so that the
+        // register sets stay registers).
+        auto iter = [&](auto ic, int m) {
+            constexpr int I = decltype(ic)::value;
+            constexpr int S = (I + 2) % NS;
+            if (m + 2 < nChunks) mstore(R[S], m + 2);
+            if (m + 2 + NS < nChunks) mload(R[S], m + 2 + NS);
+            if (m >= 1) finish(A0, A1, m - 1);
+            if (m + 1 < nChunks) {
+                if constexpr (I % 2 == 0)
+                    aload(A1, m + 1);
+                else
+                    aload(A0, m + 1);
+            }
+            wgBarrier();
+        };
         int k = 0;
-        for (; k + 1 < nChunks; k += 2) {
-            if (k + 2 < nChunks) mstore(R0, k + 2);
-            if (k + 4 < nChunks) mload(R0, k + 4);
-            if (k >= 1) finish(A0, A1, k - 1);
-            aload(A1, k + 1);
-            wgBarrier();
-            if (k + 3 < nChunks) mstore(R1, k + 3);
-            if (k + 5 < nChunks) mload(R1, k + 5);
-            finish(A0, A1, k);
-            if (k + 2 < nChunks) aload(A0, k + 2);
-            wgBarrier();
+        for (; k + NS <= nChunks; k += NS) {
+            auto sub = [&](auto ic) { iter(ic, k + decltype(ic)::value); };
+            unrollSteps<NS>(sub);
         }
-        if (k < nChunks) {
-            if (k >= 1) finish(A0, A1, k - 1);
-            wgBarrier();
-        }
+        auto tail = [&](auto ic) {
+            if (k + decltype(ic)::value < nChunks) iter(ic, k + decltype(ic)::value);
+        };
+        unrollSteps<NS>(tail);
         finish(A0, A1, nChunks - 1);
         return;
     }
+
+    // The recurrence wave shares its SIMD with another workgroup's memory
+    // wave, which runs chunks ahead; when both have an instruction ready the
+    // recurrence's dependent chain goes first.
+    __builtin_amdgcn_s_setprio(3);
 
     // Path state of this lane's labels: complemented H (fast form) or L with
     // the reference's activity flag (EXACT).
@@ -858,17 +848,18 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
     // Pixel step u of a chunk.  cw: complemented costs; p2: this step's P2'
     // (line-uniform); shv: per-step window shifts of this lane's line (slot u is
     // read on moved steps only); bigMask bit u: some line shifts by more than 1;
-    // mk: MODR lane masks of the chunk (byte u of dword u / 4).
+    // mw: MODR mask words of the chunk (lane r16 of a line's row holds step
+    // r16's).
     //
     // MODR: lane g holds the label whose real disparity r is congruent to g
     // modulo 16, so a window move changes no register: the previous pixel's
     // path cost at real r is this lane's own value and those at r - 1, r + 1
     // are the row rotations by one lane, each valid iff that real was in the
-    // previous pixel's window (masks m0, mm, mp, precomputed per step by the
-    // memory wave).  Invalid operands read 0 ("no path"), which is what
+    // previous pixel's window (masks m0, mm, mp: bits of two words per line
+    // and step that the memory wave computes).  Invalid operands read 0 ("no path"), which is what
     // sgm_kernels.cu:1237-1243 gives them (SHRT_MAX before the complement).
     auto step = [&](auto uc, const uint32_t (&cw)[(K + 1) / 2], int p2, bool moved,
-                    uint32_t bigMask, int shv, char* tp, const u4v* mk) {
+                    uint32_t bigMask, int shv, char* tp, const int* mw) {
         constexpr int u = decltype(uc)::value;
         int e[K];
 #pragma unroll
@@ -883,10 +874,15 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
                 (void)moved;
                 (void)bigMask;
                 (void)shv;
-                constexpr int B = u & 3, Q = u >> 2;
-                lowN = andByteMask<B>(__builtin_amdgcn_mov_dpp(Hs[0], kDppRowRor1, 0xF, 0xF, false), mk[1][Q]);
-                upN = andByteMask<B>(__builtin_amdgcn_mov_dpp(Hs[0], kDppRowRor15, 0xF, 0xF, false), mk[2][Q]);
-                h0[0] = andByteMask<B>(Hs[0], mk[0][Q]);
+                // Step u's mask words from lane u of the line's row; bit g
+                // sign-extended is this lane's 0 / -1 mask.
+                const int w01 = rowBcast<u>(mw[0]), wp = rowBcast<u>(mw[1]);
+                const int m0 = __builtin_amdgcn_sbfe(w01, g, 1);
+                const int mm = __builtin_amdgcn_sbfe(w01, g + 16, 1);
+                const int mp = __builtin_amdgcn_sbfe(wp, g, 1);
+                lowN = __builtin_amdgcn_mov_dpp(Hs[0], kDppRowRor1, 0xF, 0xF, false) & mm;
+                upN = __builtin_amdgcn_mov_dpp(Hs[0], kDppRowRor15, 0xF, 0xF, false) & mp;
+                h0[0] = Hs[0] & m0;
                 y[0] = imax(lowN, upN);
             } else {
                 lowN = lowerOr0<G>(Hs[K - 1], g);
@@ -977,7 +973,7 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
         u4v q[TL::CQ];
         int w;
         int w2;  // paired lines: the second line's word
-        u4v mk[MODR ? 3 : 1];  // MODR lane masks (m0, mm, mp)
+        int mw[MODR ? 2 : 1];  // MODR mask words of step r16 of line j (M0 | MM << 16, MP)
     };
     // This lane's costs of chunk k (and its row's per-step words) from LDS.
     auto cread = [&](CBuf& c, int k) {
@@ -985,9 +981,9 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
 #pragma unroll
         for (int i = 0; i < TL::CQ; ++i) c.q[i] = *reinterpret_cast<const u4v*>(p + 16 * i);
         if constexpr (MODR) {
-            const char* const pm = s_lds + TL::OFF_M + (k % 3) * TL::MSLOT + TL::laneOff(lane);
-#pragma unroll
-            for (int i = 0; i < 3; ++i) c.mk[i] = *reinterpret_cast<const u4v*>(pm + i * TL::CSLOT);
+            const char* const pm = s_lds + TL::OFF_M + (k % 3) * TL::MSLOT + (j * 16 + r16) * 4;
+            c.mw[0] = *reinterpret_cast<const int*>(pm);
+            c.mw[1] = *reinterpret_cast<const int*>(pm + TL::WSLOT);
         }
         if constexpr (WORDS) {
             const char* const wbase = s_lds + TL::OFF_W + (k % 3) * TL::WSLOT;
@@ -1173,7 +1169,7 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
                     tb = (u * LPW + j) * VB + g * K * ESZ;
                 else
                     tb = j * TL::ROWH + (RL ? (full ? U - 1 - u : nSteps - 1 - u) : u) * VB + g * K * ESZ;
-                step(uc, cw, p2, moved, bigMask, shv, tile + tb, c.mk);
+                step(uc, cw, p2, moved, bigMask, shv, tile + tb, c.mw);
             }
         };
         unrollSteps<U>(run);

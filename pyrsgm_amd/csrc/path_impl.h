@@ -206,6 +206,14 @@ __device__ __forceinline__ int gmaxPartial(int v)
     return v;
 }
 
+// v, with the compiler's knowledge of how it was formed cut off (it is
+// neither moved into a later block nor re-associated with its users).
+__device__ __forceinline__ int opaqueV(int v)
+{
+    asm("" : "+v"(v));
+    return v;
+}
+
 template <int G>
 __device__ __forceinline__ void lastPair(int p, int& a, int& b)
 {
@@ -880,10 +888,15 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
                 const int m0 = __builtin_amdgcn_sbfe(w01, g, 1);
                 const int mm = __builtin_amdgcn_sbfe(w01, g + 16, 1);
                 const int mp = __builtin_amdgcn_sbfe(wp, g, 1);
-                lowN = __builtin_amdgcn_mov_dpp(Hs[0], kDppRowRor1, 0xF, 0xF, false) & mm;
-                upN = __builtin_amdgcn_mov_dpp(Hs[0], kDppRowRor15, 0xF, 0xF, false) & mp;
+                lowN = dmov<kDppRowRor1>(Hs[0]) & mm;   // v_and_b32_dpp
+                upN = dmov<kDppRowRor15>(Hs[0]) & mp;
                 h0[0] = Hs[0] & m0;
                 y[0] = imax(lowN, upN);
+            } else if constexpr (G == 16 && K == 1) {
+                // y = v_max_i32_dpp(Hs >> 1 row, Hs << 1 row); a moved step
+                // forms its own neighbours.
+                h0[0] = Hs[0];
+                y[0] = opaqueV(imax(lowerOr0<G>(Hs[0], g), upperOr0<G>(Hs[0], g)));
             } else {
                 lowN = lowerOr0<G>(Hs[K - 1], g);
                 upN = upperOr0<G>(Hs[0], g);
@@ -899,7 +912,11 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
                     // re-indexed values are row shifts of the previous ones (zero
                     // outside the 16 labels), selected per line.
                     const int sh = rowBcast<u>(shv);
-                    const int tm2 = dmov<kDppRowShr2>(Hs[0]), tp2 = dmov<kDppRowShl2>(Hs[0]);
+                    // (Its own moves: the unmoved step's fold into its maximum.)
+                    const int hv = opaqueV(Hs[0]);
+                    lowN = lowerOr0<G>(hv, g);
+                    upN = upperOr0<G>(hv, g);
+                    const int tm2 = dmov<kDppRowShr2>(hv), tp2 = dmov<kDppRowShl2>(hv);
                     const bool neg = sh < 0, pos = sh > 0;
                     h0[0] = neg ? lowN : (pos ? upN : Hs[0]);
                     const int lm = neg ? tm2 : (pos ? Hs[0] : lowN);
@@ -926,7 +943,9 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
             int m = Hc[0];
 #pragma unroll
             for (int k = 1; k < K; ++k) m = imax(m, Hc[k]);
-            pmaxH = gmaxPartial<G>(m);
+            // (Formed here, in this step's block, so that the last level's move
+            // folds into its maximum.)
+            pmaxH = opaqueV(gmaxPartial<G>(m));
 #pragma unroll
             for (int k = 0; k < K; ++k) Hs[k] = Hc[k];
         } else {

@@ -582,6 +582,50 @@ def test_bt_window_cost_local_is_box_of_full_volume(block, cap, minD):
     np.testing.assert_array_equal(got, want)
 
 
+def test_bt_window_cost_local_caps_at_shrt_max_minus_one():
+    """A large window of high BT costs overflows the int16 range: the local
+    window cost is the plain sum capped at SHRT_MAX - 1 (sgm_kernels.cu:1105-1108;
+    SHRT_MAX itself marks a label that does not exist).  The sums are formed
+    here in int64 from the per-pixel BT volume, window by window."""
+    r = rng(77)
+    h, w, D, maxD, block, cap = 9, 40, 16, 24, 15, 63
+    left = (r.integers(0, 2, (h, w)) * 255).astype(np.uint8)
+    right = (r.integers(0, 2, (h, w)) * 255).astype(np.uint8)
+    base = np.clip(r.integers(-2, 3, (h, w)).cumsum(axis=1) + 4, 0, maxD - D + 1).astype(np.int16)
+    valid = np.ones((h, w), np.uint8)
+    full = O.bt_cost(left, right, maxD + 1, 0, cap).astype(np.int64)   # [y][x][real]
+    half = block // 2
+    ok, real = geom_valid(base, valid, D, 0, maxD)
+    k = np.clip(real, 0, maxD)
+    sums = np.zeros((h, w, D), np.int64)
+    ys, xs = np.arange(h), np.arange(w)
+    for dy in range(-half, half + 1):
+        yy = np.clip(ys + dy, 0, h - 1)
+        for dx in range(-half, half + 1):
+            xx = np.clip(xs + dx, 0, w - 1)
+            sums += np.take_along_axis(full[yy][:, xx], k, axis=2)
+    want = np.where(ok, np.minimum(sums, MAX - 1), MAX)
+    got = O.bt_window_cost_local(left, right, base, valid, D, maxD, block, 0, cap)
+    assert (sums[ok] > MAX - 1).any()          # the case reaches the cap
+    np.testing.assert_array_equal(got, want)
+
+
+def test_pyramid_rl_wta_three_geometric_labels():
+    """Near the left image edge a pixel has only x + 1 labels that project into
+    the right image; with exactly three (labels 0..2) a best at 0 and an
+    alternative at 2 make a valid pixel (validCount >= 3, sgm_kernels.cu:1420)."""
+    h, w, D, maxD = 1, 30, 16, 20
+    base = np.zeros((h, w), np.int16)
+    valid = np.ones((h, w), np.uint8)
+    C = np.full((h, w, D), 20, np.int16)
+    C[..., 0] = 0
+    C[..., 2] = 10
+    img = np.zeros((h, w), np.uint8)
+    out = O.path_rl_pyr_wta(C, base, valid, img, 0, 0, maxD, uniq=0, min_conf=0.0)
+    assert out.disp[0, 2] == 0 and out.conf[0, 2] > 0    # x = 2: labels 0, 1, 2 project
+    assert out.disp[0, 1] == -16                          # x = 1: two labels, no alternative
+
+
 # -------------------------- pyramid recurrences: real-disparity textbook
 def textbook_pyramid(C, base, valid, P1, P2, minD, maxD, direction, lut=None, img=None):
     """Pyramid path costs along one direction ('lr', 'rl' or 'tb') in

@@ -993,6 +993,11 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
         int w;
         int w2;  // paired lines: the second line's word
         int mw[MODR ? 2 : 1];  // MODR mask words of step r16 of line j (M0 | MM << 16, MP)
+        // Formed from w by prep(), half a chunk before the chunk runs (the
+        // P2' table read then completes while the previous chunk computes):
+        int p2v;               // ADAPT: this lane's step's P2' (lane r16 = step r16)
+        int shv;               // shifting refinement step: window shift of step r16
+        uint32_t moveMask, bigMask;  // bit u: some line moves at step u / by more than one
     };
     // This lane's costs of chunk k (and its row's per-step words) from LDS.
     auto cread = [&](CBuf& c, int k) {
@@ -1137,33 +1142,43 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
         unrollSteps<U>(run);
     };
 
-    // Process chunk k; full (a compile-time constant, so that full chunks carry
-    // no per-step guard): all U steps exist.
-    auto process = [&](const CBuf& c, int k, auto fullC) {
-        constexpr bool full = decltype(fullC)::value;
-        const int sc = chunkStart(k);
-        int p2v = 0, shv = 0;
-        uint32_t moveMask = 0, bigMask = 0;
-        if constexpr (REFINE && !MODR) {
+    // Per-chunk values from the chunk's per-step words (chunks in order: the
+    // carries hold the previous chunk's last step).
+    auto prep = [&](CBuf& c) {
+        c.p2v = 0;
+        c.shv = 0;
+        c.moveMask = 0;
+        c.bigMask = 0;
+        if constexpr (REFINE && !MODR && !PK) {
             const int base = (int)(int16_t)(c.w & 0xFFFF);
             const int prevBase = dpp<kDppRowShr1>(carryBase, base);
-            shv = base - prevBase;
+            c.shv = base - prevBase;
             // Bit u: some line of the wave moves at step u (any shift / |shift| > 1).
             auto fold = [](uint64_t b) {
                 const uint64_t f2 = b | (b >> 32);
                 return (uint32_t)(f2 | (f2 >> 16)) & 0xFFFFu;
             };
-            moveMask = fold(__ballot(shv != 0));
-            bigMask = fold(__ballot(shv > 1 || shv < -1));
+            c.moveMask = fold(__ballot(c.shv != 0));
+            c.bigMask = fold(__ballot(c.shv > 1 || c.shv < -1));
             carryBase = rowBcast<U - 1>(base);
         }
-        if constexpr (ADAPT) {
+        if constexpr (ADAPT && !PK) {
             const int im4 = REFINE ? (int)((uint32_t)c.w >> 22) : c.w << 2;
             const int prevIm4 = dpp<kDppRowShr1>(carryIm4, im4);
             const uint32_t d4 = __builtin_amdgcn_sad_u16((uint32_t)im4, (uint32_t)prevIm4, 0u);
-            p2v = *reinterpret_cast<const int*>(reinterpret_cast<const char*>(s_lut) + d4);
+            c.p2v = *reinterpret_cast<const int*>(reinterpret_cast<const char*>(s_lut) + d4);
             carryIm4 = rowBcast<U - 1>(im4);
         }
+    };
+
+    // Process chunk k; full (a compile-time constant, so that full chunks carry
+    // no per-step guard): all U steps exist.  mid() runs after the first half
+    // of the steps (the next chunk's prep).
+    auto process = [&](const CBuf& c, int k, auto fullC, auto&& mid) {
+        constexpr bool full = decltype(fullC)::value;
+        const int sc = chunkStart(k);
+        const int p2v = c.p2v, shv = c.shv;
+        const uint32_t moveMask = c.moveMask, bigMask = c.bigMask;
         const int nSteps = full ? U : imin(U, s1 - sc);
         char* const tile = s_lds + TL::OFF_T + (k & 1) * TL::TSLOT;
         auto run = [&](auto uc) {
@@ -1191,35 +1206,46 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
                 step(uc, cw, p2, moved, bigMask, shv, tile + tb, c.mw);
             }
         };
-        unrollSteps<U>(run);
+        unrollSteps<U / 2>(run);
+        mid();
+        unrollSteps<U, U / 2>(run);
     };
 
-    auto proc = [&](const CBuf& c, int k, auto fullC) {
-        if constexpr (PK)
+    auto proc = [&](const CBuf& c, int k, auto fullC, auto&& mid) {
+        if constexpr (PK) {
             processPK(c, k, fullC);
-        else
-            process(c, k, fullC);
+            mid();
+        } else {
+            process(c, k, fullC, mid);
+        }
     };
-    // Chunk k+1 is read from LDS while chunk k is processed; barrier k+1 ends
-    // chunk k (its tile complete, slot k free for the memory wave).
+    // Chunk k+1 is read from LDS while chunk k is processed, and prepared
+    // half-way through it; barrier k+1 ends chunk k (its tile complete, slot k
+    // free for the memory wave).
     wgBarrier();
     CBuf cA, cB;
     cread(cA, 0);
+    prep(cA);
     int k = 0;
     for (; k + 2 <= nFull; k += 2) {
         cread(cB, k + 1);
-        proc(cA, k, std::true_type{});
+        proc(cA, k, std::true_type{}, [&] { prep(cB); });
         wgBarrier();
         if (k + 2 < nChunks) cread(cA, k + 2);
-        proc(cB, k + 1, std::true_type{});
+        proc(cB, k + 1, std::true_type{}, [&] {
+            if (k + 2 < nChunks) prep(cA);
+        });
         wgBarrier();
     }
     for (; k < nChunks; ++k) {
         if (k + 1 < nChunks) cread(cB, k + 1);
+        auto next = [&] {
+            if (k + 1 < nChunks) prep(cB);
+        };
         if (chunkFull(k))
-            proc(cA, k, std::true_type{});
+            proc(cA, k, std::true_type{}, next);
         else
-            proc(cA, k, std::false_type{});
+            proc(cA, k, std::false_type{}, next);
         wgBarrier();
         cA = cB;
     }

@@ -43,6 +43,9 @@ struct BatchTuning {
     int paired = -1;       ///< paired lines for 16-label byte volumes: -1 by launch size, 0 off, 1 on
     int xcd_remap = -1;    ///< XCD-contiguous T->B column groups: -1 default, 0 off, 1 on
     int side_stream = 1;   ///< refinement descriptor passes on a side stream (batches of 2+)
+    int modular = -1;      ///< refinement path step for 16-label byte volumes: -1 by launch
+                           ///< size, 0 shifting (per-step window re-index), 1 modular (lane g
+                           ///< holds disparity g mod 16, no branch)
     int fused = -1;        ///< R->L lines finish the winner-take-all (16-label byte-volume
                            ///< pyramid levels): -1 default (off), 0 off, 1 on
 };
@@ -118,16 +121,16 @@ int pyrsgm_batch_capture(void* matcher,
                          float* d_conf, size_t conf_pitch, size_t conf_stride,
                          int count, char* err, int errlen);
 int pyrsgm_batch_launch(void* matcher, hipStream_t stream, char* err, int errlen);
-/* BatchTuning as {census_rows, paired, xcd_remap, side_stream, fused}. */
-int pyrsgm_batch_set_tuning(void* matcher, const int tuning[5], char* err, int errlen);
-int pyrsgm_batch_get_tuning(void* matcher, int tuning[5]);
+/* BatchTuning as {census_rows, paired, xcd_remap, side_stream, fused, modular}. */
+int pyrsgm_batch_set_tuning(void* matcher, const int tuning[6], char* err, int errlen);
+int pyrsgm_batch_get_tuning(void* matcher, int tuning[6]);
 int pyrsgm_batch_tune(void* matcher,
                       const unsigned char* d_left, size_t left_pitch, size_t left_stride,
                       const unsigned char* d_right, size_t right_pitch, size_t right_stride,
                       short* d_disp16, size_t disp16_pitch, size_t disp16_stride,
                       float* d_disp32, size_t disp32_pitch, size_t disp32_stride,
                       float* d_conf, size_t conf_pitch, size_t conf_stride,
-                      hipStream_t stream, int reps, int best[5], double* best_ms, char* err,
+                      hipStream_t stream, int reps, int best[6], double* best_ms, char* err,
                       int errlen);
 size_t pyrsgm_batch_device_bytes(void* matcher);
 /* BatchMatcher::requiredBytes; 0 on invalid arguments (message in err). */

@@ -515,17 +515,18 @@ class BatchMatcher:
         if lib().pyrsgm_batch_launch(self._h, s.cuda_stream, err, 512) != 0:
             raise RuntimeError(err.value.decode())
 
-    _TUNING = ("census_rows", "paired", "xcd_remap", "side_stream", "fused")
+    _TUNING = ("census_rows", "paired", "xcd_remap", "side_stream", "fused", "modular")
 
-    def set_tuning(self, census_rows=4, paired=-1, xcd_remap=-1, side_stream=1, fused=-1):
+    def set_tuning(self, census_rows=4, paired=-1, xcd_remap=-1, side_stream=1, fused=-1,
+                   modular=-1):
         """Launch shapes (every choice gives the same bits); drops a captured graph."""
-        t = (C.c_int * 5)(census_rows, paired, xcd_remap, side_stream, fused)
+        t = (C.c_int * 6)(census_rows, paired, xcd_remap, side_stream, fused, modular)
         err = C.create_string_buffer(512)
         if lib().pyrsgm_batch_set_tuning(self._h, t, err, 512) != 0:
             raise ValueError(err.value.decode())
 
     def tuning(self) -> dict:
-        t = (C.c_int * 5)()
+        t = (C.c_int * 6)()
         lib().pyrsgm_batch_get_tuning(self._h, t)
         return dict(zip(self._TUNING, list(t)))
 
@@ -535,7 +536,7 @@ class BatchMatcher:
         import torch
         s = stream if stream is not None else torch.cuda.current_stream()
         args = self._args(left, right, disp16, disp32, conf, None)[:-1]
-        best, ms = (C.c_int * 5)(), C.c_double()
+        best, ms = (C.c_int * 6)(), C.c_double()
         err = C.create_string_buffer(512)
         if lib().pyrsgm_batch_tune(self._h, *args, s.cuda_stream, reps, best, C.byref(ms),
                                    err, 512) != 0:

@@ -77,7 +77,7 @@ pyrsgm_params_c fromParams(const pyrsgm::Params& p)
 } // namespace
 
 namespace {
-pyrsgm::BatchTuning batchTuning(const int t[5])
+pyrsgm::BatchTuning batchTuning(const int t[6])
 {
     pyrsgm::BatchTuning b;
     b.census_rows = t[0];
@@ -85,16 +85,18 @@ pyrsgm::BatchTuning batchTuning(const int t[5])
     b.xcd_remap = t[2];
     b.side_stream = t[3];
     b.fused = t[4];
+    b.modular = t[5];
     return b;
 }
 
-void batchTuningOut(const pyrsgm::BatchTuning& b, int t[5])
+void batchTuningOut(const pyrsgm::BatchTuning& b, int t[6])
 {
     t[0] = b.census_rows;
     t[1] = b.paired;
     t[2] = b.xcd_remap;
     t[3] = b.side_stream;
     t[4] = b.fused;
+    t[5] = b.modular;
 }
 } // namespace
 
@@ -260,7 +262,7 @@ int pyrsgm_batch_launch(void* matcher, hipStream_t stream, char* err, int errlen
 }
 
 
-int pyrsgm_batch_set_tuning(void* matcher, const int tuning[5], char* err, int errlen)
+int pyrsgm_batch_set_tuning(void* matcher, const int tuning[6], char* err, int errlen)
 {
     if (!matcher || !tuning) return -1;
     try {
@@ -272,7 +274,7 @@ int pyrsgm_batch_set_tuning(void* matcher, const int tuning[5], char* err, int e
     return -1;
 }
 
-int pyrsgm_batch_get_tuning(void* matcher, int tuning[5])
+int pyrsgm_batch_get_tuning(void* matcher, int tuning[6])
 {
     if (!matcher || !tuning) return -1;
     batchTuningOut(static_cast<pyrsgm::BatchMatcher*>(matcher)->tuning(), tuning);
@@ -285,7 +287,7 @@ int pyrsgm_batch_tune(void* matcher,
                       short* d_disp16, size_t disp16_pitch, size_t disp16_stride,
                       float* d_disp32, size_t disp32_pitch, size_t disp32_stride,
                       float* d_conf, size_t conf_pitch, size_t conf_stride,
-                      hipStream_t stream, int reps, int best[5], double* best_ms, char* err,
+                      hipStream_t stream, int reps, int best[6], double* best_ms, char* err,
                       int errlen)
 {
     if (!matcher) return -1;

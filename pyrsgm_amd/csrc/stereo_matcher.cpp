@@ -128,6 +128,8 @@ void checkBatchTuning(const BatchTuning& t)
         throw std::invalid_argument("pyrsgm: BatchTuning.side_stream must be 0 or 1");
     if (t.fused < -1 || t.fused > 1)
         throw std::invalid_argument("pyrsgm: BatchTuning.fused must be -1, 0 or 1");
+    if (t.modular < -1 || t.modular > 1)
+        throw std::invalid_argument("pyrsgm: BatchTuning.modular must be -1, 0 or 1");
 }
 
 } // namespace
@@ -156,6 +158,7 @@ struct BatchMatcher::Impl : MatcherCore {
         tiles.xcdRemap = t.xcd_remap < 0 ? 0 : (t.xcd_remap ? 2 : 1);
         useSide = t.side_stream != 0;
         tiles.fused = t.fused < 0 ? 0 : (t.fused ? 2 : 1);
+        tiles.modular = t.modular < 0 ? 0 : (t.modular ? 2 : 1);
     }
 
     double timePasses(BatchBuffer<const uint8_t> L, BatchBuffer<const uint8_t> R,
@@ -202,6 +205,8 @@ struct BatchMatcher::Impl : MatcherCore {
             if (xr != best.xcd_remap) { BatchTuning t = best; t.xcd_remap = xr; consider(t); }
         for (int fu : {-1, 0, 1})
             if (fu != best.fused) { BatchTuning t = best; t.fused = fu; consider(t); }
+        for (int md : {-1, 0, 1})
+            if (md != best.modular) { BatchTuning t = best; t.modular = md; consider(t); }
         if (side)
             for (int sd : {0, 1})
                 if (sd != best.side_stream) { BatchTuning t = best; t.side_stream = sd; consider(t); }

@@ -191,22 +191,25 @@ def test_batch_paired_lines_equal_single_pairs(n, w, h, D, levels):
     U.assert_same("pair 1 vs oracle", d16[1].cpu().numpy(), ref.disp16)
 
 
-@pytest.mark.parametrize("census_rows,paired,xcd_remap,side_stream,fused",
-                         [(2, 1, 0, 0, 1), (8, 0, 1, 1, 1), (4, -1, -1, 1, -1), (2, -1, 1, 0, 0),
-                          (4, 1, -1, 1, 0), (8, -1, 0, 1, 1)])
-def test_batch_tunings_equal_single_pairs(census_rows, paired, xcd_remap, side_stream, fused):
+@pytest.mark.parametrize("census_rows,paired,xcd_remap,side_stream,fused,modular",
+                         [(2, 1, 0, 0, 1, -1), (8, 0, 1, 1, 1, 1), (4, -1, -1, 1, -1, 0),
+                          (2, -1, 1, 0, 0, 1), (4, 1, -1, 1, 0, 0), (8, -1, 0, 1, 1, -1),
+                          (4, 0, -1, 1, 0, 0), (4, 0, -1, 1, 1, 0)])
+def test_batch_tunings_equal_single_pairs(census_rows, paired, xcd_remap, side_stream, fused,
+                                          modular):
     """Every launch shape (census rows per block, paired lines forced on or off,
-    XCD remap, side stream, winner-take-all fused into the R->L lines or not)
-    gives the same bits as the pairs matched alone."""
+    XCD remap, side stream, winner-take-all fused into the R->L lines or not,
+    shifting or modular refinement step) gives the same bits as the pairs
+    matched alone."""
     n, w, h, D = 3, 241, 135, 128
     lefts, rights = pairs(n, w, h, D - 1, seed0=900)
     params = P.validate_params(P.Params(num_disparities=D, pyramid_levels=2,
                                         cost_mode=P.CostMode.Census, p1=8, p2=32,
                                         adaptive_p2=True, median_filter=True))
     bm = P.BatchMatcher(w, h, n, params)
-    bm.set_tuning(census_rows, paired, xcd_remap, side_stream, fused)
+    bm.set_tuning(census_rows, paired, xcd_remap, side_stream, fused, modular)
     assert bm.tuning() == {"census_rows": census_rows, "paired": paired, "xcd_remap": xcd_remap,
-                           "side_stream": side_stream, "fused": fused}
+                           "side_stream": side_stream, "fused": fused, "modular": modular}
     print("case: batch tuning", bm.tuning(), flush=True)
     d16, d32, cf = run_batch(bm, batch_tensor(lefts, torch.uint8), batch_tensor(rights, torch.uint8),
                              n, h, w)
@@ -273,6 +276,9 @@ def test_batch_tune_keeps_results():
     assert ms > 0 and bm.tuning() == best and best["census_rows"] in (2, 4, 8)
     with pytest.raises(ValueError):
         bm.set_tuning(census_rows=3)
+    with pytest.raises(ValueError):
+        bm.set_tuning(modular=2)
+    bm.set_tuning(**best)
     d16, d32, cf = run_batch(bm, L, R, n, h, w)
     want = singles(lefts, rights, params)
     for i in range(n):

@@ -234,9 +234,11 @@ constexpr size_t kPairedWaves = 4096;
 // Whether a level call runs the fused winner-take-all (TileConfig::fused,
 // volume form and stages).
 int fusedWTA(const LevelCall& c, int exact);
-// Per-launch 16-label byte-volume refinement path launches of at most this many
-// workgroups use the modular step (measured: faster at 510, slower at 1020).
-constexpr size_t kModularMaxBlocks = 768;
+// 16-label byte-volume refinement path launches of at most this many line
+// groups use the modular step (two groups per workgroup; measured faster at
+// 1020 groups: 1920x1080 124 against 155 us, and slower at 2040: two 1920x1080
+// pairs 231 against 193 us).
+constexpr size_t kModularMaxBlocks = 1536;
 // Workgroups of one queued launch that are resident at once per CU (occupancy).
 int queuedBlocksPerCU(int mode, int Dp, int adaptive, int exact, int narrow, int paired);
 hipError_t launchPathsQueued(const QueueArgs& q, int mode, int Dp, int adaptive, int exact,

@@ -440,7 +440,7 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
     constexpr int VB = GK * ESZ;  // bytes of one pixel's label vector
     constexpr int W32 = TL::W32;
 
-    const bool memWave = threadIdx.x >= 64;
+    const bool memWave = (threadIdx.x & 64) != 0;  // waves 1, 3: memory; 0, 2: recurrence
     const int lane = threadIdx.x & 63;
     const int j = lane / G;
     const int g = lane - j * G;
@@ -455,6 +455,14 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
         s1 = imin(a.endX, W);
     }
     if (s0 >= s1) return;
+    if (nValid <= 0) {
+        // A workgroup's spare line group (past the last line of its direction):
+        // both waves keep the workgroup's barriers in step, one per chunk plus
+        // the first, as a working pair does.
+        const int nc = (s1 - s0 + U - 1) / U;
+        for (int c = 0; c <= nc; ++c) wgBarrier();
+        return;
+    }
 
     // Stream layouts: offset(row, col) = row * rowPitch + col * elem.
     const uint32_t volRow = (uint32_t)W * VB;
@@ -1257,29 +1265,43 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
 // each XCD fetching the line.  (Row paths read whole lines per chunk and keep
 // the dispatch order, which spreads their long lines evenly over the XCDs.)
 
+// Line groups (recurrence + memory wave pairs) per workgroup of kPaths.  The
+// byte-volume launches run two: waves 0 and 2 are the recurrences, and the
+// hardware places a workgroup's even waves on different SIMDs, so no two
+// recurrence waves share a SIMD (with one group per workgroup, a few percent
+// of them did, ran at half speed and set the launch's length).
+template <bool NARROW, bool PK>
+constexpr int pathGroupsPerBlock() { return NARROW && !PK ? 2 : 1; }
+
 template <int G, int K, int MODE, bool ADAPT, bool EXACT, bool NARROW, bool PK = false,
           bool MOD = false>
-__global__ __launch_bounds__(128) void kPaths(PathArgs a)
+__global__ __launch_bounds__((128 * pathGroupsPerBlock<NARROW, PK>())) void kPaths(PathArgs a)
 {
+    constexpr int GPB = pathGroupsPerBlock<NARROW, PK>();
+    using TL = Lds<G, K, NARROW, PK, modularRefine<G, K, MODE, EXACT, NARROW, PK, MOD>()>;
+    static_assert(TL::BYTES % 16 == 0, "each group's LDS region starts 16-byte aligned");
     __shared__ int s_lut[256];
-    __shared__ int s_xchg[MODE == kRefine ? 64 * K : 1];
-    __shared__ __attribute__((aligned(16))) char
-        s_lds[Lds<G, K, NARROW, PK, modularRefine<G, K, MODE, EXACT, NARROW, PK, MOD>()>::BYTES];
+    __shared__ int s_xchg[GPB][MODE == kRefine ? 64 * K : 1];
+    __shared__ __attribute__((aligned(16))) char s_lds[GPB][TL::BYTES];
     if constexpr (ADAPT) {
-        for (int i = threadIdx.x; i < 256; i += 128) s_lut[i] = a.lut[i];
+        for (int i = threadIdx.x; i < 256; i += 128 * GPB) s_lut[i] = a.lut[i];
         __syncthreads();
     }
+    // Block b runs line groups b * GPB + h of one direction (h = threadIdx.x /
+    // 128); a.nLRBlocks / a.nRLBlocks count blocks.
     const int b = blockIdx.x;
+    const int h = GPB > 1 ? (int)(threadIdx.x >> 7) : 0;
     const size_t pr = blockIdx.y;  // pair
     if (b < a.nLRBlocks)
-        runLine<G, K, MODE, ADAPT, EXACT, NARROW, PK, kLR, MOD>(a, pr, b, s_lut, s_xchg, s_lds);
+        runLine<G, K, MODE, ADAPT, EXACT, NARROW, PK, kLR, MOD>(a, pr, b * GPB + h, s_lut, s_xchg[h],
+                                                                s_lds[h]);
     else if (b < a.nLRBlocks + a.nRLBlocks)
-        runLine<G, K, MODE, ADAPT, EXACT, NARROW, PK, kRL, MOD>(a, pr, b - a.nLRBlocks, s_lut, s_xchg,
-                                                                s_lds);
+        runLine<G, K, MODE, ADAPT, EXACT, NARROW, PK, kRL, MOD>(a, pr, (b - a.nLRBlocks) * GPB + h, s_lut,
+                                                                s_xchg[h], s_lds[h]);
     else {
         const int t = b - a.nLRBlocks - a.nRLBlocks, nTB = (int)gridDim.x - a.nLRBlocks - a.nRLBlocks;
         runLine<G, K, MODE, ADAPT, EXACT, NARROW, PK, kTB, MOD>(
-            a, pr, a.xcdRemap ? xcdRemap(t, nTB) : t, s_lut, s_xchg, s_lds);
+            a, pr, (a.xcdRemap ? xcdRemap(t, nTB) : t) * GPB + h, s_lut, s_xchg[h], s_lds[h]);
     }
 }
 
@@ -1387,20 +1409,27 @@ hipError_t launchDirs(const PathArgs& a, bool withRL, hipStream_t s)
             return hipGetLastError();
         }
     }
-    if constexpr (NARROW && MODE == kRefine && !EXACT) {
-        // The modular step has no per-step branch but a heavier data-moving
-        // wave; it is faster up to about 3 recurrence waves per 4 SIMDs.
-        if (a.Dp == 16 && (a.modularMode == 2 ||
-                           (a.modularMode == 0 && (size_t)blocks * a.batch.n <= kModularMaxBlocks))) {
-            kPaths<16, 1, MODE, ADAPT, EXACT, true, false, true><<<grid, 128, 0, s>>>(b);
-            return hipGetLastError();
-        }
-    }
     if constexpr (NARROW) {
+        // Two line groups per block (pathGroupsPerBlock); a direction with an
+        // odd number of groups leaves its last block's second group spare.
+        constexpr int GPB = pathGroupsPerBlock<true, false>();
+        PathArgs b2 = b;
+        b2.nLRBlocks = (b.nLRBlocks + GPB - 1) / GPB;
+        b2.nRLBlocks = (b.nRLBlocks + GPB - 1) / GPB;
+        const dim3 grid2(b2.nLRBlocks + b2.nRLBlocks + (nTB + GPB - 1) / GPB, a.batch.n);
+        if constexpr (MODE == kRefine && !EXACT) {
+            // The modular step has no per-step branch but a heavier data-moving
+            // wave; it is faster up to about 1.5 recurrence waves per SIMD.
+            if (a.Dp == 16 && (a.modularMode == 2 ||
+                               (a.modularMode == 0 && (size_t)blocks * a.batch.n <= kModularMaxBlocks))) {
+                kPaths<16, 1, MODE, ADAPT, EXACT, true, false, true><<<grid2, 128 * GPB, 0, s>>>(b2);
+                return hipGetLastError();
+            }
+        }
         switch (a.Dp) {
-        case 16: kPaths<16, 1, MODE, ADAPT, EXACT, true><<<grid, 128, 0, s>>>(b); break;
-        case 32: kPaths<32, 1, MODE, ADAPT, EXACT, true><<<grid, 128, 0, s>>>(b); break;
-        case 64: kPaths<64, 1, MODE, ADAPT, EXACT, true><<<grid, 128, 0, s>>>(b); break;
+        case 16: kPaths<16, 1, MODE, ADAPT, EXACT, true><<<grid2, 128 * GPB, 0, s>>>(b2); break;
+        case 32: kPaths<32, 1, MODE, ADAPT, EXACT, true><<<grid2, 128 * GPB, 0, s>>>(b2); break;
+        case 64: kPaths<64, 1, MODE, ADAPT, EXACT, true><<<grid2, 128 * GPB, 0, s>>>(b2); break;
         default: return hipErrorInvalidValue;
         }
         return hipGetLastError();

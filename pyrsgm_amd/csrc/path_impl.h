@@ -440,7 +440,9 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
     constexpr int VB = GK * ESZ;  // bytes of one pixel's label vector
     constexpr int W32 = TL::W32;
 
-    const bool memWave = (threadIdx.x & 64) != 0;  // waves 1, 3: memory; 0, 2: recurrence
+    // Waves 1, 3: memory; 0, 2: recurrence.  Read as a scalar: the compiler
+    // does not know that a wave's threads agree.
+    const bool memWave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x & 64)) != 0;
     const int lane = threadIdx.x & 63;
     const int j = lane / G;
     const int g = lane - j * G;
@@ -1290,7 +1292,9 @@ __global__ __launch_bounds__((128 * pathGroupsPerBlock<NARROW, PK>())) void kPat
     // Block b runs line groups b * GPB + h of one direction (h = threadIdx.x /
     // 128); a.nLRBlocks / a.nRLBlocks count blocks.
     const int b = blockIdx.x;
-    const int h = GPB > 1 ? (int)(threadIdx.x >> 7) : 0;
+    // Wave-uniform, read as a scalar so that the line group's addresses (buffer
+    // descriptors among them) stay scalar.
+    const int h = GPB > 1 ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 7)) : 0;
     const size_t pr = blockIdx.y;  // pair
     if (b < a.nLRBlocks)
         runLine<G, K, MODE, ADAPT, EXACT, NARROW, PK, kLR, MOD>(a, pr, b * GPB + h, s_lut, s_xchg[h],

@@ -401,15 +401,17 @@ constexpr bool modularRefine()
 __device__ __forceinline__ u4v rotDown16(const u4v& x, int R)
 {
     const bool q1 = (R & 4) != 0, q2 = (R & 8) != 0;
-    u4v y, z;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) y[i] = q1 ? x[(i + 1) & 3] : x[i];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) z[i] = q2 ? y[(i + 2) & 3] : y[i];
-    u4v o;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) o[i] = __builtin_amdgcn_alignbyte(z[(i + 1) & 3], z[i], (uint32_t)(R & 3));
-    return o;
+    // Two stages of four selects each, on named scalars kept opaque: a select
+    // between elements of one vector or array becomes an element read at a
+    // variable index, a 4-way select per element (about twice the instructions).
+    auto opq = [](uint32_t v) { return (uint32_t)opaqueV((int)v); };
+    const uint32_t e0 = opq(x[0]), e1 = opq(x[1]), e2 = opq(x[2]), e3 = opq(x[3]);
+    const uint32_t y0 = opq(q1 ? e1 : e0), y1 = opq(q1 ? e2 : e1), y2 = opq(q1 ? e3 : e2),
+                   y3 = opq(q1 ? e0 : e3);
+    const uint32_t z0 = q2 ? y2 : y0, z1 = q2 ? y3 : y1, z2 = q2 ? y0 : y2, z3 = q2 ? y1 : y3;
+    const uint32_t b = (uint32_t)(R & 3);
+    return u4v{__builtin_amdgcn_alignbyte(z1, z0, b), __builtin_amdgcn_alignbyte(z2, z1, b),
+               __builtin_amdgcn_alignbyte(z3, z2, b), __builtin_amdgcn_alignbyte(z0, z3, b)};
 }
 
 // MODE: kSingle, kCoarse or kRefine.
@@ -567,22 +569,29 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
             const int base = (int)(int16_t)(m.w[0] & 0xFFFF);
             const int prevBase = dppOpaque<kDppRowShr1>(mcarry, base);
             mcarry = rowBcast<U - 1>(base);
-            const int sh = imin(imax(base - prevBase, -17), 17);
+            const int sh = base - prevBase;
             const int rot = base & 15;
             info = rot;
             // Lane masks of step r of line j (this lane's word): the previous
             // pixel's path cost for real base + o + e (label o, e = 0, -1, +1)
-            // exists iff o + sh + e lies in [0, 15]; label o is held by
-            // compute lane (o + rot) & 15.
-            auto range = [](int lo, int hi) {
-                const int a = imin(imax(lo, 0), 16), b = imin(imax(hi + 1, 0), 16);
-                return b > a ? ((1u << b) - 1u) & ~((1u << a) - 1u) : 0u;
+            // exists iff o + sh + e lies in [0, 15], i.e. label o is in
+            //   f(s) = 0xFFFF >> s (s >= 0), (0xFFFF << -s) & 0xFFFF (s < 0),
+            //          0 (|s| >= 16),  s = sh + e,
+            // the bits of 0xFFFF0000 >> (16 + s) for s clamped to [-16, 16].
+            // Label o is held by compute lane (o + rot) & 15: both 16-bit halves
+            // of lanes(s) are f(s) rotated left by rot (f duplicated into both
+            // halves, then a 32-bit rotation right by 16 - rot).
+            auto lanes = [&](int e) {
+                const uint32_t k = (uint32_t)imin(imax(sh + e + 16, 0), 32);
+                const uint32_t f = (uint32_t)(0xFFFF0000ull >> k);
+                const uint32_t d = __builtin_amdgcn_perm(f, f, 0x01000100u);
+                return __builtin_amdgcn_alignbit(d, d, (uint32_t)(16 - rot));
             };
-            auto toLanes = [&](uint32_t v) { return ((v | (v << 16)) >> (16 - rot)) & 0xFFFFu; };
             char* const pm = s_lds + TL::OFF_M + (k % 3) * TL::MSLOT + lane * 4;
-            *reinterpret_cast<uint32_t*>(pm) =
-                toLanes(range(-sh, 15 - sh)) | (toLanes(range(1 - sh, 16 - sh)) << 16);
-            *reinterpret_cast<uint32_t*>(pm + TL::WSLOT) = toLanes(range(-1 - sh, 14 - sh));
+            // M0 | MM << 16, and MP (its high half repeats the low one; the
+            // recurrence reads bits 0-15 only).
+            *reinterpret_cast<uint32_t*>(pm) = (lanes(0) & 0xFFFFu) | (lanes(-1) & 0xFFFF0000u);
+            *reinterpret_cast<uint32_t*>(pm + TL::WSLOT) = lanes(1);
         }
 #pragma unroll
         for (int i = 0; i < TL::NQ; ++i) {

@@ -395,7 +395,8 @@ def mixed_passes(P, torch, log, passes=8):
     io = [(dev_pairs[id(pair)][0], dev_pairs[id(pair)][1], outs[j], None, None)
           for j, (k, pair) in enumerate(reqs)]
     mb.capture(io)
-    t = _event_ms(torch, mb.launch, reps=passes - 3, warm=3)
+    # Exactly `passes` replays (the profile tables divide by it): no timed warm-up.
+    t = _event_ms(torch, mb.launch, reps=passes - 3, warm=3, warm_s=0.0)
     out = {"workload": "64 uneven pairs, one MixedBatch graph pass", "ms_per_pass": round(t, 3),
            "passes": passes, "tuning": mb.tuning(), "plan": mb.plan()}
     mb.close()

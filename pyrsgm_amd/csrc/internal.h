@@ -5,6 +5,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <type_traits>
 #include <vector>
 
 namespace pyr {
@@ -38,7 +39,10 @@ struct Batch {
 template <class T>
 __host__ __device__ __forceinline__ T* pairAt(T* p, size_t pair, size_t stride)
 {
-    return p ? reinterpret_cast<T*>(reinterpret_cast<uintptr_t>(p) + pair * stride) : p;
+    // Byte arithmetic on the pointer itself (not through an integer), so that
+    // the compiler keeps the address space it infers for p.
+    using Byte = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+    return p ? reinterpret_cast<T*>(reinterpret_cast<Byte*>(p) + pair * stride) : p;
 }
 
 // Device workspace of one (W, H, D) geometry for n pairs.  One allocation,

@@ -31,19 +31,22 @@ __device__ __forceinline__ void wtaBlock(const PathArgs& a, size_t pr, size_t bx
     const int np = a.numPaths;
     // The pixel's label word is loaded with the volumes (pc is in bounds), so
     // that the wave waits on memory once, not again after the label search.
-    const int lwPre = MODE == kRefine ? labels[pc] : 0;
+    const int lwPre = MODE == kRefine ? wta::ldGlobal(labels + pc) : 0;
 
     const int d0 = sub * 16;
     wta::Pick k;
     if constexpr (NARROW) {
-        const uint4 vr = *reinterpret_cast<const uint4*>(reinterpret_cast<const uint8_t*>(Arl) + off);
-        const uint4 vl = np >= 2 ? *reinterpret_cast<const uint4*>(reinterpret_cast<const uint8_t*>(Alr) + off)
-                                 : make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
-        const uint4 vt = np >= 3 ? *reinterpret_cast<const uint4*>(reinterpret_cast<const uint8_t*>(Atb) + off)
-                                 : make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
-        const uint32_t wr[4] = {vr.x, vr.y, vr.z, vr.w};
-        const uint32_t wl[4] = {vl.x, vl.y, vl.z, vl.w};
-        const uint32_t wt[4] = {vt.x, vt.y, vt.z, vt.w};
+        typedef uint32_t v4 __attribute__((ext_vector_type(4)));
+        auto vec = [&](const int16_t* v) {
+            return wta::ldGlobal(reinterpret_cast<const v4*>(reinterpret_cast<const uint8_t*>(v) + off));
+        };
+        const v4 none = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
+        const v4 vr = vec(Arl);
+        const v4 vl = np >= 2 ? vec(Alr) : none;
+        const v4 vt = np >= 3 ? vec(Atb) : none;
+        const uint32_t wr[4] = {vr[0], vr[1], vr[2], vr[3]};
+        const uint32_t wl[4] = {vl[0], vl[1], vl[2], vl[3]};
+        const uint32_t wt[4] = {vt[0], vt[1], vt[2], vt[3]};
         k = wta::searchNarrow<Q>(wr, wl, wt, d0, D);
     } else {
         k = wta::searchWide<Q, PYR>(Arl, Alr, Atb, off, np, d0, D);

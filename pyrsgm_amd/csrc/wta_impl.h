@@ -25,6 +25,25 @@
 namespace pyr {
 namespace wta {
 
+// Global-memory accesses through pointers held in an argument block: the
+// compiler cannot infer their address space, and a generic (flat) access also
+// counts against the LDS counter, so a wave waits on both.
+template <class T>
+__device__ __forceinline__ T ldGlobal(const T* p)
+{
+    return *(const __attribute__((address_space(1))) T*)p;
+}
+template <class T>
+__device__ __forceinline__ void stGlobal(T* p, T v)
+{
+    *(__attribute__((address_space(1))) T*)p = v;
+}
+__device__ __forceinline__ void minGlobal(uint32_t* p, uint32_t v)
+{
+    __hip_atomic_fetch_min((__attribute__((address_space(1))) uint32_t*)p, v, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // The label search of one pixel: best (smallest) summed cost and its label,
 // the best cost outside labels best-1 .. best+1, and the costs at best-1,
 // best, best+1 (SHRT_MAX where a label does not exist).
@@ -122,9 +141,10 @@ __device__ __forceinline__ Pick searchNarrow(const uint32_t (&wr)[4], const uint
 // Path volumes hold H = SHRT_MAX - L; returns the 16 path costs L.
 __device__ __forceinline__ void load16(int (&v)[16], const int16_t* p)
 {
-    const uint4 a = reinterpret_cast<const uint4*>(p)[0];
-    const uint4 b = reinterpret_cast<const uint4*>(p)[1];
-    const uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    typedef uint32_t v4 __attribute__((ext_vector_type(4)));
+    const v4 a = ldGlobal(reinterpret_cast<const v4*>(p));
+    const v4 b = ldGlobal(reinterpret_cast<const v4*>(p) + 1);
+    const uint32_t w[8] = {a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3]};
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
         v[2 * i] = kMax - (int)(w[i] & 0xFFFFu);
@@ -226,8 +246,7 @@ __device__ __forceinline__ void resolve(const PathArgs& a, const Pick& k, int x,
             cf = fminf(fmaxf(((float)alt - (float)minCost) / denom, 0.0f), 1.0f);
             const int x2 = x - bestD - a.minD;
             if (x2 >= 0 && x2 < W)
-                atomicMin(&keys[(size_t)y * W + x2],
-                          ((uint32_t)(minCost + 32768) << 16) | (uint32_t)(0xFFFF - x));
+                minGlobal(&keys[(size_t)y * W + x2], ((uint32_t)(minCost + 32768) << 16) | (uint32_t)(0xFFFF - x));
         }
     } else {
         int base = a.minD, valid = 1;
@@ -259,12 +278,11 @@ __device__ __forceinline__ void resolve(const PathArgs& a, const Pick& k, int x,
             cf = c;
             const int x2 = x - realBest;
             if (x2 >= 0 && x2 < W)
-                atomicMin(&keys[(size_t)y * W + x2],
-                          ((uint32_t)(minCost + 32768) << 16) | (uint32_t)(0xFFFF - x));
+                minGlobal(&keys[(size_t)y * W + x2], ((uint32_t)(minCost + 32768) << 16) | (uint32_t)(0xFFFF - x));
         }
     }
-    dispV[p] = out;
-    confV[p] = cf;
+    stGlobal(dispV + p, out);
+    stGlobal(confV + p, cf);
 }
 
 } // namespace wta

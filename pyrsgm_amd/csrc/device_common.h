@@ -29,6 +29,8 @@ enum : int {
     kDppRowRor1 = 0x121,       // lane i <- lane (i-1) mod 16 inside its row
     kDppRowRor15 = 0x12F,      // lane i <- lane (i+1) mod 16 inside its row
     kDppRowHalfMirror = 0x141, // lane i <- lane 7-i inside its half row
+    kDppRowBcast15 = 0x142,    // rows 1-3 <- lane 15 of the row before (with a row mask)
+    kDppRowBcast31 = 0x143,    // rows 2-3 <- lane 31 (with a row mask)
 };
 
 // DPP move; lanes whose source is outside the row/wave keep `old`.

@@ -185,13 +185,16 @@ __device__ __forceinline__ int gmax(int v)
     return v;
 }
 
-// gmax split before its last cross-row level: for 32- and 64-lane lines
-// gmaxPartial leaves, in every lane, the maximum over half of its line and
-// lastPair gives that value and the other half's (a permlane swap), so that
-// gmax(v) = max(a, b); the path step folds that level into its subtraction:
+// gmax split before its last cross-row level: for 32-lane lines gmaxPartial
+// leaves, in every lane, the maximum over half of its line and lastPair gives
+// that value and the other half's (a permlane swap), so that gmax(v) =
+// max(a, b); the path step folds that level into its subtraction:
 // x - gmax(v) = min(x - a, x - b).  A 16-lane line keeps the whole reduction
 // in gmaxPartial (a = b): x - row_mirror(a) would be a v_subrev_u32_dpp,
-// whose swizzle gfx950 applies to x instead of a (see dppOpaque).
+// whose swizzle gfx950 applies to x instead of a (see dppOpaque).  A 64-lane
+// line is the whole wave: its row maxima meet in lane 63 (row_bcast:15 into
+// rows 1 and 3, then row_bcast:31 into rows 2 and 3) and the maximum is read
+// as a scalar (a = b), with no permlane swaps or their copies and wait states.
 template <int G>
 __device__ __forceinline__ int gmaxPartial(int v)
 {
@@ -200,8 +203,12 @@ __device__ __forceinline__ int gmaxPartial(int v)
     v = imax(v, dmov<kDppRowHalfMirror>(v));
     v = imax(v, dmov<kDppRowMirror>(v));
     if constexpr (G >= 64) {
-        const auto s = __builtin_amdgcn_permlane16_swap((uint32_t)v, (uint32_t)v, false, false);
-        v = imax((int)s[0], (int)s[1]);
+        // Unsigned maxima with 0 in the masked-off rows (the identity of an
+        // unsigned maximum, and v >= 0), so that each move folds into its maximum.
+        auto up = [](int x, int o) { return (int)__builtin_elementwise_max((uint32_t)x, (uint32_t)o); };
+        v = up(v, __builtin_amdgcn_update_dpp(0, v, kDppRowBcast15, 0xA, 0xF, false));
+        v = up(v, __builtin_amdgcn_update_dpp(0, v, kDppRowBcast31, 0xC, 0xF, false));
+        v = __builtin_amdgcn_readlane(v, 63);
     }
     return v;
 }
@@ -213,16 +220,21 @@ __device__ __forceinline__ int opaqueV(int v)
     asm("" : "+v"(v));
     return v;
 }
+// The same for a wave-uniform value kept in a scalar register.
+__device__ __forceinline__ int opaqueS(int v)
+{
+    asm("" : "+s"(v));
+    return v;
+}
 
 template <int G>
 __device__ __forceinline__ void lastPair(int p, int& a, int& b)
 {
-    if constexpr (G == 16) {
+    if constexpr (G == 16 || G == 64) {
         a = p;
         b = p;
     } else {
-        const auto s = G == 32 ? __builtin_amdgcn_permlane16_swap((uint32_t)p, (uint32_t)p, false, false)
-                               : __builtin_amdgcn_permlane32_swap((uint32_t)p, (uint32_t)p, false, false);
+        const auto s = __builtin_amdgcn_permlane16_swap((uint32_t)p, (uint32_t)p, false, false);
         a = (int)s[0];
         b = (int)s[1];
     }
@@ -964,7 +976,10 @@ __device__ __forceinline__ void runLine(const PathArgs& a, size_t pair, int line
             for (int k = 1; k < K; ++k) m = imax(m, Hc[k]);
             // (Formed here, in this step's block, so that the last level's move
             // folds into its maximum.)
-            pmaxH = opaqueV(gmaxPartial<G>(m));
+            if constexpr (G == 64)
+                pmaxH = opaqueS(gmaxPartial<G>(m));
+            else
+                pmaxH = opaqueV(gmaxPartial<G>(m));
 #pragma unroll
             for (int k = 0; k < K; ++k) Hs[k] = Hc[k];
         } else {

@@ -119,9 +119,9 @@ public:
     uint64_t submit(const ServeRequest& request, int64_t arrival_ns = 0);
     /// True (and the result, which is then forgotten) once the ticket is done.
     bool poll(uint64_t ticket, ServeResult* result);
-    /// Block until the ticket is done.  Throws std::invalid_argument for an unknown
-    /// ticket, and for one that another call (poll, takeResults, a second wait)
-    /// retrieves first, also while this call waits.
+    /// Block until the ticket is done.  Throws std::invalid_argument for a ticket
+    /// never issued ("unknown ticket"), and for one that another call (poll,
+    /// takeResults, a second wait) retrieved, before or while this call waits.
     ServeResult wait(uint64_t ticket);
     /// Block until every submitted request is done.
     void drain();

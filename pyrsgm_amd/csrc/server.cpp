@@ -582,8 +582,13 @@ bool Server::poll(uint64_t ticket, ServeResult* result)
 ServeResult Server::wait(uint64_t ticket)
 {
     std::unique_lock<std::mutex> lk(impl_->mu);
-    if (!impl_->finished.count(ticket) && !impl_->inflight.count(ticket))
+    if (!impl_->finished.count(ticket) && !impl_->inflight.count(ticket)) {
+        // Tickets are issued in increasing order from 1: an issued ticket that
+        // is neither in flight nor finished was retrieved already.
+        if (ticket != 0 && ticket < impl_->nextTicket)
+            throw std::invalid_argument("pyrsgm: ticket was retrieved by another call");
         throw std::invalid_argument("pyrsgm: unknown ticket");
+    }
     // A ticket leaves `inflight` only into `finished`, and leaves `finished`
     // when some caller (poll, takeResults, another wait) retrieves it: the
     // wait ends in either case.

@@ -916,9 +916,10 @@ __device__ __forceinline__ int reflect101(int i, int n)
     return i < 0 ? 0 : i;
 }
 
-// Block: 64 x 8 destination pixels of one image, two rows per thread; the
-// 131 x 19 source window they read (reflect-101 coordinates) is loaded into
-// registers with every load in flight at once, then staged in LDS.
+// Block: 64 x 8 destination pixels of one image, two adjacent rows per thread;
+// the 131 x 19 source window they read (reflect-101 coordinates) is loaded into
+// registers with every load in flight at once, then staged in LDS (rows padded
+// to 132 bytes, so each row starts on a dword).
 constexpr int kPyrRows = 8;
 
 __device__ __forceinline__ void kPyrDownBody(Tile3 tl, const uint8_t* srcA, int sstepA,
@@ -927,7 +928,8 @@ __device__ __forceinline__ void kPyrDownBody(Tile3 tl, const uint8_t* srcA, int 
                                              int dstepB, size_t dStride, int dw, int dh, int nv)
 {
     constexpr int TW = 131, TH = 2 * kPyrRows + 3, TILE = TW * TH, NL = (TILE + 255) / 256;
-    __shared__ uint8_t s_t[TH][TW + 1];
+    __shared__ __attribute__((aligned(16))) uint8_t s_t[TH][TW + 1];
+    static_assert((TW + 1) % 4 == 0, "rows start on a dword");
     const size_t pr = tl.z / nv;              // pair
     const int view = (int)(tl.z - pr * nv);   // 0: A, 1: B
     const uint8_t* src = pairAt(view ? srcB : srcA, pr, sStride);
@@ -953,18 +955,28 @@ __device__ __forceinline__ void kPyrDownBody(Tile3 tl, const uint8_t* srcA, int 
     if (x >= dw) return;
     uint8_t* dst = pairAt(view ? dstB : dstA, pr, dStride);
     const int dstep = view ? dstepB : dstepA;
-    constexpr int w[5] = {1, 4, 6, 4, 1};
+    // Two vertically adjacent outputs per thread (rows 2rg, 2rg+1 of the
+    // block): the 5-tap row sums of the 7 source rows they share are formed
+    // once from two dword reads per row; the column filter then combines 5 of
+    // them per output.  The separable integer sums equal the 5x5 ones.
+    const int ly = 2 * (threadIdx.x >> 6);
+    const int off = 2 * (lx & 1);  // byte offset of column 2lx in its dword
+    int hs[7];
 #pragma unroll
-    for (int rr = 0; rr < 2; ++rr) {
-        const int ly = (threadIdx.x >> 6) + 4 * rr;
-        const int y = y0 + ly;
+    for (int r = 0; r < 7; ++r) {
+        const uint32_t* row = reinterpret_cast<const uint32_t*>(s_t[2 * ly + r]) + (lx >> 1);
+        const uint32_t d0 = row[0], d1 = row[1];
+        const uint32_t u = __builtin_amdgcn_alignbyte(d1, d0, (uint32_t)off);  // bytes 2lx .. 2lx+3
+        const int b4 = (int)((d1 >> (8 * off)) & 0xFFu);                         // byte 2lx+4
+        hs[r] = (int)(u & 0xFFu) + b4 + 4 * ((int)((u >> 8) & 0xFFu) + (int)(u >> 24)) +
+                6 * (int)((u >> 16) & 0xFFu);
+    }
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int y = y0 + ly + k;
         if (y >= dh) break;
-        int sum = 0;
-#pragma unroll
-        for (int ky = 0; ky < 5; ++ky) {
-            const uint8_t* row = s_t[2 * ly + ky] + 2 * lx;
-            sum += w[ky] * (row[0] + 4 * row[1] + 6 * row[2] + 4 * row[3] + row[4]);
-        }
+        const int* h = hs + 2 * k;
+        const int sum = h[0] + h[4] + 4 * (h[1] + h[3]) + 6 * h[2];
         dst[(size_t)y * dstep + x] = (uint8_t)((sum + 128) >> 8);
     }
 }
